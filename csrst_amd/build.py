@@ -1,0 +1,64 @@
+"""Build the gfx950 kernel library libcsrst.so in-tree with hipcc.
+
+The library is one shared object with the C ABI declared in
+include/csrst_hip.h.  Device code is compiled without FMA contraction
+(-ffp-contract=off) so the FP64 step is bit-identical to the reference
+arithmetic; the FP32 kernels carry their own explicit fused operations where
+they are wanted.  Packed-FP32 instruction selection is disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcsrst.so")
+SOURCES = ["csrst_ref.hip", "csrst_mw.hip", "csrst_poisson.hip", "csrst_unet.hip"]
+ARCH = os.environ.get("CSRST_ARCH", "gfx950")
+
+
+def hipcc():
+    for c in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.exists(c) or c == "hipcc":
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def flags():
+    return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+            "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-result", "-I" + CSRC, "-I" + os.path.join(HERE, "..", "include")]
+
+
+def build(verbose=True, jobs=4):
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = []
+    procs = []
+    for s in srcs:
+        obj = os.path.join(CSRC, s.replace(".hip", ".o"))
+        cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, s), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), s))
+        objs.append(obj)
+        if len(procs) >= jobs:
+            p, name = procs.pop(0)
+            if p.wait() != 0:
+                raise RuntimeError("compile failed: " + name)
+    for p, name in procs:
+        if p.wait() != 0:
+            raise RuntimeError("compile failed: " + name)
+    tmp = LIB + ".tmp%d" % os.getpid()
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

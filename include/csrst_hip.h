@@ -1,0 +1,131 @@
+/*
+ * csrst_hip.h -- C ABI of the MI355X (gfx950) implementation of CSRST's
+ * particle-in-cell time step.
+ *
+ * Plain pointers and sizes only; every call returns 0 on success or a
+ * negative CSRST_E* code, and csrst_last_error() returns the message.  No
+ * exception crosses the ABI.  A context is used from one host thread; its
+ * GPU work is ordered on the context's stream.
+ *
+ * Two steppers sit behind this ABI:
+ *  - csrst_ctx: the reference's two-grid step (global masked chamber grid +
+ *    bunch-following window), FP64, bit-identical to the reference
+ *    arithmetic.  It replaces TETMSolver::updateField_firstStep/secondStep
+ *    (TETMSolver.hh:110-266) with PartBunch::push_particles
+ *    (PartBunch.cpp:215-519), the ShapeFunction<IntCIC|IntS1|IntS2> deposits
+ *    and the window bookkeeping of main.cpp:390-506, 562-652.
+ *  - csrst_mw: the production moving-window step (one W x H grid that
+ *    follows the bunch and carries the conformal chamber walls), FP64 or
+ *    FP32 with split-precision particle positions.
+ */
+#ifndef CSRST_HIP_H
+#define CSRST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSRST_OK 0
+#define CSRST_EINVAL (-1)
+#define CSRST_ENOMEM (-2)
+#define CSRST_EHIP (-3)
+#define CSRST_EWINDOW (-4)   /* bunch left the window / window move outside {0,1}x{-1,0,1} */
+#define CSRST_EGEOM (-5)     /* geometry set-up failed (crossBoundary out of range) */
+#define CSRST_ESTATE (-6)    /* call order violated (e.g. step before init) */
+
+typedef struct csrst_ctx csrst_ctx;
+typedef struct csrst_mw csrst_mw;
+
+/* Commands input of main.cpp:111-136 that shapes the step. */
+typedef struct {
+    int Nx, Ny;              /* -x/--numgridpx, -y/--numgridpy: global grid */
+    double lenBefore;        /* --lengthbefore [m] */
+    double lenAfter;         /* -l/--length [m] */
+    double width;            /* -w/--width [m] */
+    double phiDeg;           /* -P/--phi [deg] */
+    double ekinMeV;          /* -E/--Ekin [MeV] */
+    double bz;               /* -B/--Bz [T] */
+    int order;               /* shape function: 0 IntCIC (defs.hh:55), 1 IntS1, 2 IntS2 */
+    int s2bug;               /* 1: IntS2 gather reads (I+1, J+2) as at ScatterGatherImpl.hh:467,525 */
+    int statModulus;         /* --statmodulus: wake sampled every statModulus steps */
+} csrst_params;
+
+const char *csrst_version(void);
+const char *csrst_last_error(const void *ctx_or_null);
+int csrst_device_count(int *n);
+
+/* ---- two-grid reference step ------------------------------------------ */
+
+/* Allocates the global fields (AllocateFields.cpp:57-120), builds the Bend
+ * geometry and the conformal boundary cells (Bend.cpp:85-353,
+ * SCBoundaryCondition.hh:96-137).  main.cpp:168-197. */
+int csrst_create(const csrst_params *p, csrst_ctx **out);
+int csrst_destroy(csrst_ctx *c);
+
+/* Mesh spacing, origin and time step: out[0..1] hr, out[2..3] origin,
+ * out[4] dt = hr0/c (main.cpp:187), out[5] R, out[6] phi. */
+int csrst_geometry(csrst_ctx *c, double *out);
+/* Inside mask (Nx*Ny bytes) and the boundary-cell list: ij (2 ints per cell),
+ * data (lambda0, lambda1, area, lambdaN0, lambdaN1 per cell). */
+int csrst_boundary_cells(csrst_ctx *c, uint8_t *mask, int *ij, double *data, int cap, int *count);
+
+/* Bunch set-up of the -I start: window = bunch box +- 5 cells
+ * (main.cpp:199-217), PartBunch sort (PartBunch.cpp:1817-1844),
+ * makeReady(dt/2) (PartBunch.cpp:164-177), global->window copy and
+ * boundary exchange (main.cpp:335-347).  R, P: 2 doubles per particle. */
+int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, int n);
+
+/* Whole steps (main.cpp:390-506). */
+int csrst_step(csrst_ctx *c, int nsteps);
+/* Fine-grained parts of one step, for bisection against the CPU oracle. */
+int csrst_first_step(csrst_ctx *c);            /* TETMSolver.hh:110-211 */
+int csrst_second_step(csrst_ctx *c);           /* TETMSolver.hh:213-266 */
+int csrst_move_window(csrst_ctx *c, int *moveI, int *moveJ);   /* main.cpp:494-501 */
+
+/* which: 0 global E, 1 global H, 2 window E, 3 window H, 4 window J, 5 global J.
+ * Planar layout [2][m+2][n+2] over [-1, n] x [-1, m] (guard layer included).
+ * *count receives the number of doubles. */
+int csrst_download_field(csrst_ctx *c, int which, double *out, int cap, int *count);
+int csrst_upload_field(csrst_ctx *c, int which, const double *in, int count);
+int csrst_download_particles(csrst_ctx *c, double *R, double *P, double *oldR, int n);
+/* info: dt, window origin x, y, window position x, y, W, H, steps, wake samples */
+int csrst_info(csrst_ctx *c, double *info);
+/* CSR wake accumulator of IntegrateElongCommand (IntegrateElong.cpp:38-89). */
+int csrst_wake(csrst_ctx *c, double *out, int cap, int *count);
+/* rmean(2), pmean(2), qtotal, field energy of the global grid */
+int csrst_diag(csrst_ctx *c, double *out);
+
+/* ---- moving-window production step -------------------------------------- */
+
+typedef struct {
+    int W, H;                /* window cells */
+    double dx, dy;           /* mesh spacing [m] */
+    double lenBefore, lenAfter, width, phiDeg, ekinMeV, bz;   /* chamber and bend */
+    int order;               /* 0, 1, 2 */
+    int precision;           /* 0: FP64, 1: FP32 fields + int32 cell / FP32 offset positions */
+    int sortEvery;           /* cell sort period in steps (0: never) */
+    int statModulus;         /* wake sampling period */
+    double cfl;              /* c*dt/dx; 1.0 is the reference's dt = hx/c */
+} csrst_mw_params;
+
+int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out);
+int csrst_mw_destroy(csrst_mw *m);
+/* Positions R in metres (global chamber frame), P = beta*gamma, Q [C]. */
+int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const double *Q, int n);
+int csrst_mw_step(csrst_mw *m, int nsteps);
+/* planar [6][H+2][W+2] doubles: Ex, Ey, Hzx, Hzy, Jx, Jy (guards incl.) */
+int csrst_mw_download_fields(csrst_mw *m, double *out, int cap, int *count);
+int csrst_mw_download_particles(csrst_mw *m, double *R, double *P, int n);
+/* info: dt, origin x, y, column offset, row offset, W, H, steps, wake samples, n */
+int csrst_mw_info(csrst_mw *m, double *info);
+int csrst_mw_wake(csrst_mw *m, double *out, int cap, int *count);
+/* Average device time of each kernel family over the last csrst_mw_step call:
+ * [0] push+deposit, [1] field sweeps, [2] window move, [3] sort, [4] wake (ms). */
+int csrst_mw_timing(csrst_mw *m, double *ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

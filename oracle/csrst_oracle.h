@@ -1,0 +1,100 @@
+/*
+ * csrst_oracle.h -- CPU FP64 oracle for the CSRST particle-in-cell time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain C++ restatement of the
+ * reference algorithm (kreischtauf/CSRST, src/) and exists to check the HIP
+ * product path.  Only tests/, the smoke() checker and the CPU-baseline leg of
+ * bench.py may load it.  The product library never links, loads or calls it.
+ *
+ * Build: g++ -O2 -ffp-contract=off (no FMA contraction, like the reference's
+ * ISO C++ build, CMakeLists.txt:85-87).
+ *
+ * Pinning status of each function is listed in oracle/README.md.
+ */
+#ifndef CSRST_ORACLE_H
+#define CSRST_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- primitives ------------------------------------------------------- */
+float orc_q_rsqrt(float x);                                       /* utils.hh:491-504 */
+int   orc_crossings(int dual, const double oldR[2], const double newR[2],
+                    const double dx[2], const double origin[2],
+                    double *frac, double *sub);                   /* utils.hh:364-489 */
+
+/* ScatterGatherImpl on a dense patch (row-major, pnx fastest).
+ * order: 0 = IntCIC, 1 = IntS1, 2 = IntS2.  s2bug selects the reference's
+ * (I+1, J+2) index for the last IntS2 gather term (ScatterGatherImpl.hh:467). */
+void orc_gather_edge(int order, int comp, int s2bug, const double *patch, int pnx, int pny,
+                     const double origin[2], const double dx[2],
+                     const double *R, int n, double *out);
+void orc_gather_cell(int order, int s2bug, const double *patch, int pnx, int pny,
+                     const double origin[2], const double dx[2],
+                     const double *R, int n, double *out);
+void orc_scatter_edge(int order, int comp, double *patch, int pnx, int pny,
+                      const double origin[2], const double dx[2],
+                      const double *R, const double *val, int n);
+void orc_scatter_cell(int order, double *patch, int pnx, int pny,
+                      const double origin[2], const double dx[2],
+                      const double *R, const double *val, int n);
+
+/* Charge-conserving current deposition onto an absolute-index patch
+ * [i0, i0+pnx) x [j0, j0+pny) of a mesh with the given origin/spacing.
+ * Returns -1 if a contribution falls outside the patch. */
+int orc_deposit(int order, int comp, double *J, int i0, int j0, int pnx, int pny,
+                const double origin[2], const double dx[2], double dt,
+                const double *oldR, const double *R, const double *Q, int n);
+/* Vertex charge density (divided by hx*hy); same patch convention. */
+int orc_charge_density(int order, double *rho, int i0, int j0, int pnx, int pny,
+                       const double origin[2], const double dx[2],
+                       const double *R, const double *Q, int n);
+
+/* ---- geometry (Bend.cpp) --------------------------------------------- */
+void *orc_bend_create(double lenBefore, double lenAfter, double width, double phiDeg,
+                      double ekinMeV, double bz, int Nx, int Ny);
+void  orc_bend_destroy(void *b);
+/* out: hr0 hr1 origin0 origin1 R phi center0 center1 domain_length domain_width */
+void  orc_bend_info(void *b, double *out);
+int   orc_bend_is_inside(void *b, double x, double y);
+void  orc_bend_mask(void *b, uint8_t *mask);                       /* Nx*Ny, row-major */
+int   orc_bend_num_bcells(void *b);
+/* ij: 2 ints per cell; data: lambda0 lambda1 area lN0 lN1 per cell (after limitArea) */
+void  orc_bend_bcells(void *b, int *ij, double *data);
+void  orc_bend_external_field(void *b, const double *R, int n, double *out);
+
+/* ---- two-grid time step (main.cpp:390-506) -------------------------- */
+typedef struct {
+    int Nx, Ny;                 /* global grid (AllocateFields) */
+    double lenBefore, lenAfter, width, phiDeg, ekinMeV, bz;   /* Bend */
+    int order;                  /* 0 CIC, 1 S1, 2 S2 */
+    int s2bug;                  /* reproduce ScatterGatherImpl.hh:467 */
+    int statModulus;            /* wake sampling period (steps) */
+} orc_sim_params;
+
+void *orc_sim_create(const orc_sim_params *p);
+void  orc_sim_destroy(void *s);
+/* R, P interleaved (x,y) per particle; Q per particle.  Performs the -I start:
+ * window sizing, PartBunch sort, makeReady(dt/2), copy global->window, exchange. */
+int   orc_sim_init(void *s, const double *R, const double *P, const double *Q, int n);
+int   orc_sim_step(void *s, int nsteps);
+/* info: dt, window origin x/y, ddim first x/y, W, H, steps done, wake samples */
+void  orc_sim_info(void *s, double *info);
+/* which: 0 global E, 1 global H, 2 window E, 3 window H, 4 window J.
+ * Copies the interleaved 2-component field over [-1, n] x [-1, m] (guards incl.)
+ * Returns number of doubles written. */
+int   orc_sim_get_field(void *s, int which, double *out);
+void  orc_sim_get_particles(void *s, double *R, double *P, double *oldR);
+int   orc_sim_get_wake(void *s, double *out);
+/* compensated diagnostics: rmean(2) pmean(2) qtotal field_energy */
+void  orc_sim_diag(void *s, double *out);
+/* set a field (same layout as get) */
+int   orc_sim_set_field(void *s, int which, const double *in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

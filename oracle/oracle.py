@@ -1,0 +1,253 @@
+"""ctypes front end of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke() checker and by the
+CPU-baseline leg of bench.py.  The product package never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class SimParams(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int),
+                ("lenBefore", C.c_double), ("lenAfter", C.c_double), ("width", C.c_double),
+                ("phiDeg", C.c_double), ("ekinMeV", C.c_double), ("bz", C.c_double),
+                ("order", C.c_int), ("s2bug", C.c_int), ("statModulus", C.c_int)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_q_rsqrt.restype = C.c_float
+        L.orc_q_rsqrt.argtypes = [C.c_float]
+        L.orc_crossings.restype = C.c_int
+        L.orc_bend_create.restype = C.c_void_p
+        L.orc_bend_create.argtypes = [C.c_double] * 6 + [C.c_int, C.c_int]
+        for f in ("orc_bend_destroy", "orc_bend_info", "orc_bend_mask", "orc_bend_num_bcells",
+                  "orc_bend_bcells", "orc_bend_external_field", "orc_bend_is_inside"):
+            getattr(L, f).argtypes = None
+        L.orc_bend_is_inside.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_sim_create.restype = C.c_void_p
+        L.orc_sim_create.argtypes = [C.POINTER(SimParams)]
+        _LIB = L
+    return _LIB
+
+
+def q_rsqrt(x):
+    return lib().orc_q_rsqrt(C.c_float(x))
+
+
+def crossings(oldR, newR, dx, origin, dual=False):
+    frac = np.zeros(10)
+    sub = np.zeros(20)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (oldR, newR, dx, origin)]
+    n = lib().orc_crossings(int(dual), *[_ptr(v) for v in a], _ptr(frac), _ptr(sub))
+    return frac[:n].copy(), sub[:2 * n].reshape(n, 2).copy()
+
+
+def gather_edge(order, comp, patch, origin, dx, R, s2bug=True):
+    patch = np.ascontiguousarray(patch, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(len(R))
+    pny, pnx = patch.shape
+    lib().orc_gather_edge(C.c_int(order), C.c_int(comp), C.c_int(int(s2bug)), _ptr(patch), C.c_int(pnx),
+                          C.c_int(pny), _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                          _ptr(R), C.c_int(len(R)), _ptr(out))
+    return out
+
+
+def gather_cell(order, patch, origin, dx, R, s2bug=True):
+    patch = np.ascontiguousarray(patch, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(len(R))
+    pny, pnx = patch.shape
+    lib().orc_gather_cell(C.c_int(order), C.c_int(int(s2bug)), _ptr(patch), C.c_int(pnx), C.c_int(pny),
+                          _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                          _ptr(R), C.c_int(len(R)), _ptr(out))
+    return out
+
+
+def scatter_edge(order, comp, shape, origin, dx, R, val):
+    patch = np.zeros(shape)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 2)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    lib().orc_scatter_edge(C.c_int(order), C.c_int(comp), _ptr(patch), C.c_int(shape[1]), C.c_int(shape[0]),
+                           _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                           _ptr(R), _ptr(val), C.c_int(len(R)))
+    return patch
+
+
+def scatter_cell(order, shape, origin, dx, R, val):
+    patch = np.zeros(shape)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 2)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    lib().orc_scatter_cell(C.c_int(order), _ptr(patch), C.c_int(shape[1]), C.c_int(shape[0]),
+                           _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                           _ptr(R), _ptr(val), C.c_int(len(R)))
+    return patch
+
+
+def deposit(order, comp, box, origin, dx, dt, oldR, R, Q):
+    """box = (i0, j0, nx, ny) absolute node indices; returns J[ny, nx]."""
+    i0, j0, nx, ny = box
+    J = np.zeros((ny, nx))
+    oldR = np.ascontiguousarray(oldR, np.float64).reshape(-1, 2)
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
+    Q = np.ascontiguousarray(Q, np.float64)
+    rc = lib().orc_deposit(C.c_int(order), C.c_int(comp), _ptr(J), C.c_int(i0), C.c_int(j0), C.c_int(nx),
+                           C.c_int(ny), _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                           C.c_double(dt), _ptr(oldR), _ptr(R), _ptr(Q), C.c_int(len(R)))
+    if rc != 0:
+        raise ValueError("deposit left the patch")
+    return J
+
+
+def charge_density(order, box, origin, dx, R, Q):
+    i0, j0, nx, ny = box
+    rho = np.zeros((ny, nx))
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
+    Q = np.ascontiguousarray(Q, np.float64)
+    rc = lib().orc_charge_density(C.c_int(order), _ptr(rho), C.c_int(i0), C.c_int(j0), C.c_int(nx), C.c_int(ny),
+                                  _ptr(np.asarray(origin, np.float64)), _ptr(np.asarray(dx, np.float64)),
+                                  _ptr(R), _ptr(Q), C.c_int(len(R)))
+    if rc != 0:
+        raise ValueError("charge density left the patch")
+    return rho
+
+
+class Bend:
+    """Bend geometry (Bend.cpp) on an Nx x Ny global vertex mesh."""
+
+    def __init__(self, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, Nx, Ny):
+        L = lib()
+        self.h = L.orc_bend_create(lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, Nx, Ny)
+        if not self.h:
+            raise RuntimeError("bend geometry failed")
+        self.Nx, self.Ny = Nx, Ny
+        info = np.zeros(10)
+        L.orc_bend_info(C.c_void_p(self.h), _ptr(info))
+        self.hr = info[0:2].copy()
+        self.origin = info[2:4].copy()
+        self.R, self.phi = info[4], info[5]
+        self.center = info[6:8].copy()
+        self.domain = info[8:10].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bend_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def is_inside(self, x, y):
+        return bool(lib().orc_bend_is_inside(C.c_void_p(self.h), x, y))
+
+    def mask(self):
+        m = np.zeros((self.Ny, self.Nx), np.uint8)
+        lib().orc_bend_mask(C.c_void_p(self.h), _ptr(m, C.c_uint8))
+        return m
+
+    def bcells(self):
+        n = lib().orc_bend_num_bcells(C.c_void_p(self.h))
+        ij = np.zeros((n, 2), np.int32)
+        data = np.zeros((n, 5))
+        lib().orc_bend_bcells(C.c_void_p(self.h), _ptr(ij, C.c_int), _ptr(data))
+        return ij, data
+
+    def external_field(self, R):
+        R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
+        out = np.zeros(len(R))
+        lib().orc_bend_external_field(C.c_void_p(self.h), _ptr(R), C.c_int(len(R)), _ptr(out))
+        return out
+
+
+class Sim:
+    """Two-grid PIC time step of the reference (main.cpp:390-506), FP64 on the CPU."""
+
+    def __init__(self, Nx, Ny, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order=0, s2bug=True,
+                 statModulus=1):
+        self.p = SimParams(Nx, Ny, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order, int(s2bug), statModulus)
+        self.h = lib().orc_sim_create(C.byref(self.p))
+        if not self.h:
+            raise RuntimeError("oracle sim create failed")
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sim_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def init(self, R, P, Q):
+        R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
+        P = np.ascontiguousarray(P, np.float64).reshape(-1, 2)
+        Q = np.ascontiguousarray(Q, np.float64)
+        self.n = len(R)
+        rc = lib().orc_sim_init(C.c_void_p(self.h), _ptr(R), _ptr(P), _ptr(Q), C.c_int(self.n))
+        if rc != 0:
+            raise RuntimeError("oracle init failed (%d)" % rc)
+
+    def step(self, n=1):
+        rc = lib().orc_sim_step(C.c_void_p(self.h), C.c_int(n))
+        if rc != 0:
+            raise RuntimeError("oracle step failed")
+
+    def info(self):
+        a = np.zeros(9)
+        lib().orc_sim_info(C.c_void_p(self.h), _ptr(a))
+        return dict(dt=a[0], origin=(a[1], a[2]), pos=(int(a[3]), int(a[4])), W=int(a[5]), H=int(a[6]),
+                    steps=int(a[7]), nsamp=int(a[8]))
+
+    def field(self, which):
+        """which: 'E', 'H' (global), 'mE', 'mH', 'mJ' (window), 'J' (global).
+        Returns array [2, n+2, m+2] over [-1, n] x [-1, m] (component, j, i)."""
+        code = {"E": 0, "H": 1, "mE": 2, "mH": 3, "mJ": 4, "J": 5}[which]
+        inf = self.info()
+        if which in ("E", "H", "J"):
+            n, m = self.p.Nx, self.p.Ny
+        else:
+            n, m = inf["W"], inf["H"]
+        out = np.zeros((m + 2) * (n + 2) * 2)
+        lib().orc_sim_get_field(C.c_void_p(self.h), C.c_int(code), _ptr(out))
+        return out.reshape(m + 2, n + 2, 2).transpose(2, 0, 1).copy()
+
+    def set_field(self, which, arr):
+        code = {"E": 0, "H": 1, "mE": 2, "mH": 3, "mJ": 4, "J": 5}[which]
+        a = np.ascontiguousarray(np.asarray(arr, np.float64).transpose(1, 2, 0))
+        lib().orc_sim_set_field(C.c_void_p(self.h), C.c_int(code), _ptr(a))
+
+    def particles(self):
+        R = np.zeros((self.n, 2))
+        P = np.zeros((self.n, 2))
+        O = np.zeros((self.n, 2))
+        lib().orc_sim_get_particles(C.c_void_p(self.h), _ptr(R), _ptr(P), _ptr(O))
+        return R, P, O
+
+    def wake(self):
+        n = self.info()["nsamp"]
+        out = np.zeros(max(n, 1))
+        lib().orc_sim_get_wake(C.c_void_p(self.h), _ptr(out))
+        return out[:n]
+
+    def diag(self):
+        out = np.zeros(6)
+        lib().orc_sim_diag(C.c_void_p(self.h), _ptr(out))
+        return out
