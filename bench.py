@@ -1,0 +1,151 @@
+"""Benchmark of the CSRST PIC time step on one MI355X.
+
+Workload (default): 8192 x 2048 window at 10 um, 30 deg bend of radius 5 m,
+seeded Gaussian bunch gamma 500, 0.5 nC, sigma_s 200 um, sigma_x 100 um,
+energy spread 1e-3, 1e7 macro-particles, second-order shape function, cell
+sort every 20 steps.  The FP32 split-precision step is timed, then the same
+run in FP64; the CSR wake of both runs is compared.
+
+Prints one JSON line (the benchmark record).  A step = one full TE/TM time
+step (two pushes with deposits, the field sweeps, the window move).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_PART = {True: 48, False: 64}     # bytes per particle-step (FP32 split, FP64)
+B_CELL = {True: 48, False: 96}     # bytes per cell-update (FP32, FP64)
+
+
+def run_mw(cfg, fp32, steps, warmup, R, P, Q, stat):
+    from csrst_amd.mw import MovingWindowStep
+    m = MovingWindowStep(cfg["W"], cfg["H"], cfg["dx"], cfg["dy"], cfg["lenBefore"], cfg["lenAfter"], cfg["width"],
+                         cfg["phiDeg"], cfg["ekinMeV"], cfg["bz"], order=cfg["order"], fp32=fp32,
+                         sortEvery=cfg["sortEvery"], statModulus=stat)
+    m.init_bunch(R, P, Q)
+    if warmup:
+        m.step(warmup)
+    t0 = time.perf_counter()
+    m.step(steps)          # synchronises the library's stream before returning
+    t1 = time.perf_counter()
+    timing = m.timing()
+    wake = m.wake()
+    info = m.info()
+    m.close()
+    return (t1 - t0), timing, wake, info
+
+
+def cpu_baseline(seconds=12.0):
+    """The FP64 CPU restatement of the reference step (oracle, one core) on
+    the reference's CPU-runnable workload: global grid of the 10 deg bend at
+    ~50 um (4694 x 595), 1e5 macro-particles, IntCIC; timed for a bounded
+    number of steps."""
+    import numpy as np
+    from oracle import oracle as O
+    from csrst_amd.bunch import gaussian_bunch
+    Nx, Ny = 4694, 595
+    geo = dict(lenBefore=0.05, lenAfter=0.01, width=0.0128, phiDeg=10.0, ekinMeV=50.59, bz=0.1704)
+    s = O.Sim(Nx, Ny, order=0, statModulus=100000, **geo)
+    n = 100000
+    R, P, Q = gaussian_bunch(n, (0.03, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=1)
+    s.init(R, P, Q)
+    s.step(1)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        s.step(2)
+        steps += 2
+    t = time.perf_counter() - t0
+    return dict(value=n * steps / t, unit="particle-steps/s", cores=1, kind="port",
+                cell_updates_per_s=Nx * Ny * steps / t,
+                sample="oracle two-grid step, 4694x595 global grid + bunch window, 1e5 particles IntCIC, "
+                       "%d steps in %.1f s on one host core" % (steps, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--particles", type=int, default=0, help="override the particle count")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the PIC step on one GPU only; --gpus %d is not supported" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from csrst_amd import _lib
+    from csrst_amd.configs import config, bunch_for
+    if not _lib.has_gpu():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cfg = config(a.config)
+    n = a.particles or cfg["n"]
+    print("bench: config %s, %d particles, %dx%d cells, %d+%d steps" % (a.config, n, cfg["W"], cfg["H"], a.warmup,
+                                                                        a.steps), file=sys.stderr, flush=True)
+    R, P, Q = bunch_for(cfg, n=n, seed=1)
+    t32, tim32, wake32, info32 = run_mw(cfg, True, a.steps, a.warmup, R, P, Q, a.stat)
+    print("bench: fp32 %.3f s, timing %s" % (t32, tim32), file=sys.stderr, flush=True)
+    cells = cfg["W"] * cfg["H"]
+    ps32 = n * a.steps / t32
+    cu32 = cells * a.steps / t32
+    rec64 = None
+    if not a.no_fp64:
+        t64, tim64, wake64, info64 = run_mw(cfg, False, a.steps, a.warmup, R, P, Q, a.stat)
+        print("bench: fp64 %.3f s, timing %s" % (t64, tim64), file=sys.stderr, flush=True)
+        scale = np.abs(wake64).max()
+        werr = float(np.abs(wake32 - wake64).max() / scale) if scale > 0 else None
+        rec64 = dict(particle_steps_per_s=n * a.steps / t64, cell_updates_per_s=cells * a.steps / t64,
+                     ms_per_step=1e3 * t64 / a.steps, speedup_fp32_over_fp64=t64 / t32, wake_rel_err=werr,
+                     window_moves_match=info64["pos"] == info32["pos"], timing_ms=tim64)
+    # roofline of the dominant kernel (the particle push + deposit, two launches per step)
+    push_launch_ms = tim32["push_ms"] / 2.0
+    sweep_ms = tim32["sweep_ms"]
+    achieved = n * B_PART[True] / (push_launch_ms * 1e-3) / 1e9
+    frac_step = (cells * B_CELL[True] + n * B_PART[True]) * a.steps / t32 / 1e9 / HBM_PEAK_GBS
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline()
+    rec = {
+        "metric": "particle-steps/s + cell-updates/s on 1 MI355X, FP32 vs FP64 at wake err <=1e-4",
+        "value": ps32,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t32 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (split int32+fp32 positions)",
+        "data": "synthetic seeded Gaussian bunch (Philox), analytic bend chamber",
+        "config": {"workload": "%s: %dx%d window dx=%g um, %.0f deg bend R=%g m, gamma %g, %g nC, %d particles, "
+                               "shape S%d, sort every %d" % (a.config, cfg["W"], cfg["H"], cfg["dx"] * 1e6,
+                                                              cfg["phiDeg"], cfg["R"], cfg["gamma"],
+                                                              cfg["charge_nC"], n, cfg["order"], cfg["sortEvery"]),
+                   "cells": cells, "particles": n},
+        "cell_updates_per_s": cu32,
+        "step_roofline_frac": frac_step,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push (push + deposit, %d B per particle per launch)" % B_PART[True],
+                     "launch_ms": push_launch_ms, "sweep_ms_per_step": sweep_ms},
+        "fp64": rec64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

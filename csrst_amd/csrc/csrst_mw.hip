@@ -1,0 +1,1391 @@
+// csrst_mw.hip -- production PIC step on gfx950: one W x H window grid that
+// follows the bunch and carries the conformal chamber walls.
+//
+// Layout in HBM
+//   fields   Ex, Ey, Hz (Hz = Hzx + Hzy merged: the split only matters with
+//            PML, which the reference never applies), Jx, Jy as real planes
+//            of (W+2) x (H+2) cells (one guard layer), used as a 2-D ring
+//            buffer: moving the window by one column/row only clears the
+//            recycled guard column/row -- no field copy.
+//   geometry one byte per cell (0 no update, 1 bulk, 2 cut cell) in the same
+//            ring, plus per-cell cut data (lambda0, lambda1, area, lambdaN0,
+//            lambdaN1) read only at cut cells.  New columns/rows get their
+//            geometry from the analytic bend on the device (FP64).
+//   particles int32 global cell index + real offset in [0, 1) per axis
+//            (split precision), real momentum (beta*gamma); uniform charge.
+//
+// Kernels per step
+//   k_push<comp=0>  drift, gather, kick+rotation, drift, S0/S1/S2 deposit of
+//                   Jx (wave-uniform stencils reduced across the wavefront,
+//                   one atomic per node), drift back
+//   k_heh           H half-step, Ex, H half-step in one row-marching pass
+//   k_push<comp=1>  same, depositing Jy
+//   k_ey            Ey
+//   k_move          window move decision on the device, ring-offset update
+//   k_fresh         clear + geometry for the recycled guard column/row
+// plus a periodic cell sort and the wake sampling.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/csrst_hip.h"
+#include "bend_geom.h"
+#include "hip_util.h"
+#include "pic_math.h"
+
+using hu::DBuf;
+using hu::Error;
+using pm::vec2;
+
+namespace mw {
+
+constexpr int kB = 256;
+constexpr int G = 1;  // guard layer
+
+// Device-resident window state (changes every step without host round trips).
+struct WinState {
+    int wx0, wy0;        // global cell index of logical cell (0, 0)
+    int rx, ry;          // ring offsets
+    int mi, mj;          // last move
+    int bb[4];           // particle bounding box in global cells: min x, min y, -max x, -max y (atomicMin)
+    int jbox[2][4];      // current footprint per component (global nodes): lo x, lo y, -hi x, -hi y
+    int err;             // 1: stencil left the window, 2: too many crossings, 4: move out of range
+    int steps;
+    double p0[2];        // reference momentum: particles store P - p0 (re-centred at every sort)
+};
+
+template <class T>
+struct Fields {
+    T *ex, *ey, *hz, *jx, *jy;
+    T *ex2, *hz2;        // ping-pong partners of ex and hz (written by k_heh)
+    uint8_t *cls;
+    T *cut;              // 5 planes: lam0, lam1, area, lamN0, lamN1
+    int W, H, Wp, Hp;    // logical interior size, physical (padded) size
+    size_t plane;
+};
+
+template <class T>
+struct Parts {
+    int *gx, *gy;
+    T *fx, *fy, *px, *py;
+    int n;
+};
+
+struct Geo {
+    geom::Bend bend;
+    double org[2], h[2];  // global virtual mesh: vertex (gx, gy) at org + (gx*h0, gy*h1)
+};
+
+// logical -> physical ring index
+__device__ __forceinline__ int ringx(int i, int rx, int Wp)
+{
+    int p = i + G + rx;
+    p = p >= Wp ? p - Wp : p;
+    return p < 0 ? p + Wp : p;
+}
+__device__ __forceinline__ int ringy(int j, int ry, int Hp)
+{
+    int p = j + G + ry;
+    p = p >= Hp ? p - Hp : p;
+    return p < 0 ? p + Hp : p;
+}
+
+// ---------------------------------------------------------------------------
+// geometry of one logical cell (global index gx, gy)
+template <class T>
+__device__ void cell_geometry(const Geo &g, const Fields<T> &f, int gx, int gy, size_t q)
+{
+    geom::CutCell cut;
+    bool ok;
+    const long long big = (long long)1 << 40;
+    const int k = geom::classify(g.bend, g.org, g.h, gx, gy, big, big, &cut, &ok);
+    f.cls[q] = (uint8_t)(k == 1 ? 1 : (k == 2 ? 2 : 0));
+    if (k == 2) {
+        f.cut[q] = (T)cut.lam[0];
+        f.cut[f.plane + q] = (T)cut.lam[1];
+        f.cut[2 * f.plane + q] = (T)cut.area;
+        f.cut[3 * f.plane + q] = (T)cut.lamN[0];
+        f.cut[4 * f.plane + q] = (T)cut.lamN[1];
+    }
+}
+
+// full-window geometry + clear (set-up)
+template <class T>
+__global__ void k_geom_all(Fields<T> f, Geo g, const WinState *ws)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - G;
+    const int j = blockIdx.y - G;
+    if (i > f.W - 1 + G) return;
+    const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
+    f.ex[q] = 0; f.ey[q] = 0; f.hz[q] = 0; f.jx[q] = 0; f.jy[q] = 0; f.ex2[q] = 0; f.hz2[q] = 0;
+    cell_geometry(g, f, ws->wx0 + i, ws->wy0 + j, q);
+}
+
+// ---------------------------------------------------------------------------
+// particle push + deposit
+template <class T>
+struct PushConst {
+    T hx, hy;            // half drift in cells: 0.5 c dt / h
+    T fact1, fact2;      // kick factors (PartBunch.cpp:364-365)
+    T qodt_x, qodt_y;    // q / (h dt) per axis
+    T odt_x, odt_y;      // 1 / (h dt) per axis
+    T q;
+};
+
+// wave-wide OR / MAX of an int (64 lanes)
+__device__ __forceinline__ int wave_or(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Kick and rotation of PartBunch.cpp:366-383 on a momentum stored as
+// P = p0 + dP: the increments are formed directly so that FP32 keeps the
+// resolution of dP (|dP| << |P|).  Algebraically identical to
+// P' = ((1+a^2) s + a s_perp)/(1+a^2) + kick, s = um + a (um_y, -um_x).
+template <class T>
+__device__ __forceinline__ vec2<T> kick_rotate_split(vec2<T> p0, vec2<T> dP, T ex, T ey, T hf, T bext, T fact1, T fact2)
+{
+    const T B = T(pm::kMu0) * hf + T(0.5) * bext;
+    const vec2<T> kick = pm::mk(fact1 * ex, fact1 * ey);
+    const vec2<T> um = pm::mk(p0.x + (dP.x + kick.x), p0.y + (dP.y + kick.y));
+    const T a = fact2 * B * (T)pm::q_rsqrt((float)(T(1.0) + (um.x * um.x + um.y * um.y)));
+    const T a2 = a * a;
+    const T r = T(1.0) / (T(1.0) + a2);
+    const T c1 = a * (T(1.0) + r);
+    const T dx = c1 * um.y - a2 * r * um.x;
+    const T dy = -c1 * um.x - a2 * r * um.y;
+    return pm::mk(dP.x + ((kick.x + kick.x) + dx), dP.y + ((kick.y + kick.y) + dy));
+}
+
+template <class T>
+__device__ __forceinline__ void split_add(int &g, T &f, T d)
+{
+    f = f + d;
+    const T fl = pm::floor_(f);
+    g += (int)fl;
+    f = f - fl;
+}
+
+// Wave-level (64 lanes) sum with DPP row shifts and row broadcasts: lane 63
+// ends with the total, which readlane returns to every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xf, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffu), CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWMASK, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <class T>
+__device__ __forceinline__ T dpp(T v);
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += dpp_f<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_f<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_f<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_f<0x118, 0xf>(v);   // row_shr:8  -> lane 15 of each row holds the row sum
+    v += dpp_f<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+    v += dpp_f<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3 -> lane 63 holds the total
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_d<0x111, 0xf>(v);
+    v += dpp_d<0x112, 0xf>(v);
+    v += dpp_d<0x114, 0xf>(v);
+    v += dpp_d<0x118, 0xf>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffu), 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), 63);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <class T>
+__device__ __forceinline__ void atomic_add_real(T *p, T v);
+template <>
+__device__ __forceinline__ void atomic_add_real<float>(float *p, float v) { unsafeAtomicAdd(p, v); }
+template <>
+__device__ __forceinline__ void atomic_add_real<double>(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// Trajectory split for one push: with c dt <= hx <= hy a particle crosses at
+// most one cell (or dual-cell) line per axis, so the list has 2..4 entries
+// and the loops below have fixed trip counts (registers, no scratch).
+template <class T>
+struct Split {
+    T f[4];
+    vec2<T> s[4];
+    int n;   // entries (segments = n - 1)
+};
+
+template <class T>
+__device__ __forceinline__ T absv(T v) { return v < T(0) ? -v : v; }
+
+template <class T>
+__device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bool dual)
+{
+    const T sh = dual ? T(0.5) : T(0);
+    const int Ii = (int)pm::floor_(o.x - sh), Ji = (int)pm::floor_(o.y - sh);
+    const int If = (int)pm::floor_(e.x - sh), Jf = (int)pm::floor_(e.y - sh);
+    const int dI = If - Ii, dJ = Jf - Ji;
+    const vec2<T> D = pm::mk(e.x - o.x, e.y - o.y);
+    const T eps = T(1e-6);
+    // the (at most two) interior crossings
+    T fa = T(2), fb = T(2);
+    vec2<T> sa = o, sb = o;
+    if (dI != 0) {
+        const T xb = (T)(dI > 0 ? If : Ii) + sh;
+        fa = (xb - o.x) / D.x;
+        sa = pm::mk(xb, o.y + fa * D.y);
+    }
+    if (dJ != 0) {
+        const T yb = (T)(dJ > 0 ? Jf : Ji) + sh;
+        fb = (yb - o.y) / D.y;
+        sb = pm::mk(o.x + fb * D.x, yb);
+    }
+    const bool swp = fb < fa;
+    const T f1 = swp ? fb : fa, f2 = swp ? fa : fb;
+    const vec2<T> s1 = swp ? sb : sa, s2 = swp ? sa : sb;
+    // duplicates within 1e-6 in fraction merge into the earlier entry
+    bool v1 = f1 < T(2), v2 = f2 < T(2);
+    v1 = v1 && !(absv(f1) < eps);
+    v2 = v2 && !(absv(f2 - (v1 ? f1 : T(0))) < eps);
+    // the end point: merges into the last kept crossing if within 1e-6
+    const T flast = v2 ? f2 : (v1 ? f1 : T(0));
+    const bool endMerged = (v1 || v2) && absv(T(1) - flast) < eps;
+    // fixed slots: 0 start, then kept crossings, then the end
+    c.f[0] = T(0); c.s[0] = o;
+    c.f[1] = v1 ? f1 : (v2 ? f2 : T(1));
+    c.s[1] = v1 ? s1 : (v2 ? s2 : e);
+    c.f[2] = (v1 && v2) ? f2 : T(1);
+    c.s[2] = (v1 && v2) ? s2 : e;
+    c.f[3] = T(1); c.s[3] = e;
+    const int n = 2 + (v1 ? 1 : 0) + (v2 ? 1 : 0) - (endMerged ? 1 : 0);
+    // a merged end: the last kept crossing takes the end position (its fraction stays)
+    c.s[1] = (endMerged && n == 2) ? e : c.s[1];
+    c.s[2] = (endMerged && n == 3) ? e : c.s[2];
+    c.n = n;
+    return (dI >= -1 && dI <= 1 && dJ >= -1 && dJ <= 1);
+}
+
+constexpr int kPPT = 8;              // particles per thread in one workgroup chunk
+constexpr int kChunk = kB * kPPT;    // 2048 particles per workgroup
+constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
+
+template <class T, int ORDER, int COMP>
+__global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
+{
+    __shared__ T acc[kTH * kTW];
+    __shared__ int sbox[4];          // chunk box of old cells, then written-node box
+    __shared__ int wbox[4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int base = blockIdx.x * kChunk;
+    const int wx0 = ws->wx0, wy0 = ws->wy0, rx = ws->rx, ry = ws->ry;
+    const vec2<T> p0 = pm::mk((T)ws->p0[0], (T)ws->p0[1]);
+    const int W = f.W, H = f.H;
+    T *J = COMP == 0 ? f.jx : f.jy;
+    // ---- tile anchor: minimum old cell of the chunk
+    int mnx = INT_MAX, mny = INT_MAX;
+    for (int it = 0; it < kPPT; ++it) {
+        const int k = base + it * kB + tid;
+        if (k < p.n) { mnx = min(mnx, p.gx[k]); mny = min(mny, p.gy[k]); }
+    }
+    mnx = hu::wave_min(mnx); mny = hu::wave_min(mny);
+    if (tid == 0) { sbox[0] = INT_MAX; sbox[1] = INT_MAX; wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN; }
+    for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
+    __syncthreads();
+    if (lane == 0) { atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny); }
+    __syncthreads();
+    const int ax = sbox[0] - 2, ay = sbox[1] - 2;   // LDS node (0,0) = global node (ax, ay)
+    int err = 0;
+    int bminx = INT_MAX, bminy = INT_MAX, bmaxx = INT_MIN, bmaxy = INT_MIN;
+    int tminx = INT_MAX, tminy = INT_MAX, tmaxx = INT_MIN, tmaxy = INT_MIN;
+    constexpr int NX = ORDER == 0 ? (COMP == 0 ? 1 : 2) : (ORDER == 1 ? (COMP == 0 ? 2 : 3) : (COMP == 0 ? 3 : 4));
+    constexpr int NY = ORDER == 0 ? (COMP == 0 ? 2 : 1) : (ORDER == 1 ? (COMP == 0 ? 3 : 2) : (COMP == 0 ? 4 : 3));
+    const vec2<T> odt0 = pm::mk(c.odt_y, c.odt_x);   // S0: (1/(hy dt), 1/(hx dt)), ZerothOrderShapeFunction.cpp:224
+    const vec2<T> qodt = pm::mk(c.qodt_x, c.qodt_y);
+    for (int it = 0; it < kPPT; ++it) {
+        const int k = base + it * kB + tid;
+        const bool live = k < p.n;
+        int gx0 = 0, gy0 = 0, gx = 0, gy = 0;
+        T fx0 = 0, fy0 = 0, fx = 0, fy = 0;
+        vec2<T> dP = pm::mk(T(0), T(0)), P = dP;
+        T g2 = 0;
+        if (live) {
+            gx0 = p.gx[k]; gy0 = p.gy[k];
+            fx0 = p.fx[k]; fy0 = p.fy[k];
+            dP = pm::mk(p.px[k], p.py[k]);
+            P = pm::mk(p0.x + dP.x, p0.y + dP.y);
+            // half drift (PartBunch.cpp:228-232)
+            const T gi = pm::inv_gamma(P);
+            gx = gx0; gy = gy0; fx = fx0; fy = fy0;
+            split_add(gx, fx, c.hx * P.x * gi);
+            split_add(gy, fy, c.hy * P.y * gi);
+            // gather at R1 (positions relative to the particle's cell corner);
+            // ring offsets of the stencil rows/columns computed once per field
+            const vec2<T> pp = pm::mk(fx, fy);
+            constexpr int N = ORDER + 2;
+            auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
+                int col[N];
+                size_t rowb[N];
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    const int i = gx + (int)st.I0 + a - wx0, j = gy + (int)st.J0 + a - wy0;
+                    if (i < -G || i > W - 1 + G || j < -G || j > H - 1 + G) err |= 1;
+                    col[a] = ringx(min(max(i, -G), W - 1 + G), rx, f.Wp);
+                    rowb[a] = (size_t)ringy(min(max(j, -G), H - 1 + G), ry, f.Hp) * f.Wp;
+                }
+                return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
+            };
+            const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
+            const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
+            const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
+            const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
+            const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
+            const T bext = (T)geom::external_bz(g.bend, X, Y);
+            dP = kick_rotate_split(p0, dP, ex, ey, hf, bext, c.fact1, c.fact2);
+            P = pm::mk(p0.x + dP.x, p0.y + dP.y);
+            g2 = pm::inv_gamma(P);
+            split_add(gx, fx, c.hx * P.x * g2);
+            split_add(gy, fy, c.hy * P.y * g2);
+        }
+        // ---- deposit of R0 -> R2, cell units relative to the old cell corner
+        const vec2<T> o = pm::mk(fx0, fy0);
+        const vec2<T> e = pm::mk((T)(gx - gx0) + fx, (T)(gy - gy0) + fy);
+        Split<T> xs;
+        xs.n = 1;
+        if (live && !split_path(xs, o, e, ORDER == 1)) err |= 2;
+        const vec2<T> delta = pm::mk(e.x - o.x, e.y - o.y);
+        const int nseg = live ? xs.n - 1 : 0;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const bool act = s < nseg;
+            if (__ballot(act) == 0) continue;
+            pm::SegCurrent<T> sc;
+            sc.I0 = INT_MIN; sc.J0 = INT_MIN;
+            if (act) {
+                // slot selects instead of xs.s[s] keep the list in registers
+                const vec2<T> xi = s == 0 ? xs.s[0] : (s == 1 ? xs.s[1] : xs.s[2]);
+                const vec2<T> xf = s == 0 ? xs.s[1] : (s == 1 ? xs.s[2] : xs.s[3]);
+                const T fi = s == 0 ? xs.f[0] : (s == 1 ? xs.f[1] : xs.f[2]);
+                const T ff = s == 0 ? xs.f[1] : (s == 1 ? xs.f[2] : xs.f[3]);
+                if (ORDER == 0) {
+                    pm::seg_s0(sc, COMP, xi, xf, c.q, odt0);
+                } else {
+                    const T tau = ff - fi;
+                    const vec2<T> cc = pm::mk(T(0.5) * (xi.x + xf.x), T(0.5) * (xi.y + xf.y));
+                    const int I = ORDER == 1 ? (int)pm::floor_(cc.x + T(0.5)) : (int)pm::floor_(cc.x);
+                    const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
+                    const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
+                    const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
+                    if (ORDER == 1) pm::seg_s1(sc, COMP, tau, a, b, qodt, delta);
+                    else pm::seg_s2(sc, COMP, tau, a, b, qodt, delta);
+                    sc.I0 = I - 1; sc.J0 = Jn - 1;
+                }
+                sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
+            }
+            const unsigned long long am = __ballot(act);
+            const int lead = __ffsll((long long)am) - 1;
+            const int aI = __shfl(sc.I0, lead, 64), aJ = __shfl(sc.J0, lead, 64);
+            const bool uniform = __ballot(act && (sc.I0 != aI || sc.J0 != aJ)) == 0;
+            if (uniform) {
+                // stencil shared by the wavefront: one LDS add per node
+#pragma unroll
+                for (int t = 0; t < NX * NY; ++t) {
+                    const T v = wave_sum(act ? sc.w[t] : T(0));
+                    if (lane == lead) {
+                        const int gxn = aI + (t % NX), gyn = aJ + (t / NX);
+                        const int li = gxn - ax, lj = gyn - ay;
+                        if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
+                            atomicAdd(&acc[lj * kTW + li], v);
+                            tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+                        } else {
+                            const int i = gxn - wx0, j = gyn - wy0;
+                            if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
+                            else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
+                        }
+                    }
+                }
+            } else if (act) {
+#pragma unroll
+                for (int t = 0; t < NX * NY; ++t) {
+                    const int gxn = sc.I0 + (t % NX), gyn = sc.J0 + (t / NX);
+                    const int li = gxn - ax, lj = gyn - ay;
+                    if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
+                        atomicAdd(&acc[lj * kTW + li], sc.w[t]);
+                        tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+                    } else {
+                        const int i = gxn - wx0, j = gyn - wy0;
+                        if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { err |= 1; continue; }
+                        atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], sc.w[t]);
+                    }
+                }
+            }
+        }
+        // ---- drift back by half a step (TETMSolver.hh:210, 265) and store
+        if (live) {
+            bminx = min(bminx, min(gx0, gx)); bminy = min(bminy, min(gy0, gy));
+            bmaxx = max(bmaxx, max(gx0, gx)); bmaxy = max(bmaxy, max(gy0, gy));
+            split_add(gx, fx, -c.hx * P.x * g2);
+            split_add(gy, fy, -c.hy * P.y * g2);
+            p.gx[k] = gx; p.gy[k] = gy;
+            p.fx[k] = fx; p.fy[k] = fy;
+            p.px[k] = dP.x; p.py[k] = dP.y;
+        }
+    }
+    // ---- written-node box of the tile, then flush the tile with one global
+    // atomic per touched node
+    tminx = hu::wave_min(tminx); tminy = hu::wave_min(tminy);
+    tmaxx = hu::wave_max(tmaxx); tmaxy = hu::wave_max(tmaxy);
+    bminx = hu::wave_min(bminx); bminy = hu::wave_min(bminy);
+    bmaxx = hu::wave_max(bmaxx); bmaxy = hu::wave_max(bmaxy);
+    err = wave_or(err);
+    if (lane == 0) {
+        if (tminx != INT_MAX) {
+            atomicMin(&wbox[0], tminx); atomicMin(&wbox[1], tminy);
+            atomicMax(&wbox[2], tmaxx); atomicMax(&wbox[3], tmaxy);
+        }
+        if (bminx != INT_MAX) {
+            atomicMin(&ws->bb[0], bminx); atomicMin(&ws->bb[1], bminy);
+            atomicMin(&ws->bb[2], -bmaxx); atomicMin(&ws->bb[3], -bmaxy);
+        }
+        if (err) atomicOr(&ws->err, err);
+    }
+    __syncthreads();
+    if (wbox[0] == INT_MAX) return;
+    const int tw = wbox[2] - wbox[0] + 1, th = wbox[3] - wbox[1] + 1;
+    for (int q = tid; q < tw * th; q += kB) {
+        const int li = wbox[0] + q % tw, lj = wbox[1] + q / tw;
+        const T v = acc[lj * kTW + li];
+        if (v == T(0)) continue;
+        const int i = ax + li - wx0, j = ay + lj - wy0;
+        if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { atomicOr(&ws->err, 1); continue; }
+        atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Field sweeps.  Thread = column, marching over a strip of rows.
+//
+// k_heh: H half step, Ex, H half step (TETMSolver.hh:142-206 without the
+// exchanges, with the conformal cell updates of TETMSolver.cpp:87-115 in
+// place of the bulk update at cut cells).  For row j it computes H1(j)
+// (needs E(j+1) old), Ex(j) (needs H1(j), H1(j-1)), then H2(j-1) (needs
+// Ex(j), Ex(j-1) new).  Ey is read-only in this kernel.
+template <class T>
+struct SweepConst {
+    T sx, sy;            // 0.5 etax / Z0, 0.5 etay / Z0 (bulk H)
+    T sex, dteps;        // etay Z0 (bulk Ex), c dt Z0
+    T sey;               // etax Z0 (bulk Ey)
+    T cx, cy;            // 0.5 dt / (h mu0) (conformal H)
+    T ce;                // dt / eps0 (conformal E)
+    T ihx, ihy;          // 1/hx, 1/hy
+};
+
+constexpr int kStrip = 32;
+
+template <class T>
+__device__ __forceinline__ T h_update(const Fields<T> &f, size_t q, uint8_t cls, T h, T ey, T eyp, T ex, T exn,
+                                      const SweepConst<T> &c)
+{
+    if (cls == 1) return (h - c.sx * (eyp - ey)) + c.sy * (exn - ex);
+    if (cls == 2) {
+        const T l0 = f.cut[q], l1 = f.cut[f.plane + q], ar = f.cut[2 * f.plane + q];
+        const T n0 = f.cut[3 * f.plane + q], n1 = f.cut[4 * f.plane + q];
+        const T u0 = -c.cx * (n1 * eyp - l1 * ey) / ar;
+        const T u1 = +c.cy * (n0 * exn - l0 * ex) / ar;
+        return h + (u0 + u1);
+    }
+    return h;
+}
+
+template <class T>
+__device__ __forceinline__ T ex_update(const Fields<T> &f, size_t q, uint8_t cls, T ex, T h, T hm, T jv,
+                                       const SweepConst<T> &c)
+{
+    if (cls == 1) return ex + (c.sex * (h - hm) - c.dteps * jv);
+    if (cls == 2 && f.cut[q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea))
+        return ex + c.ce * ((h - hm) * c.ihy - jv);
+    return ex;
+}
+
+// Row march for one column i over the strip [j0, j1]: H1 for rows
+// j0-1 .. j1+1, Ex for rows j0 .. j1+1, H2 for rows j0 .. j1; only H2 and Ex
+// of the strip rows are stored, into the partner buffers (ex2, hz2), so the
+// halo rows other strips read are never overwritten during the pass.  Guard
+// rows (-1, H) are never updated.
+template <class T>
+__global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const WinState *ws, int jbase)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // logical column 0..W-1
+    if (i >= f.W) return;
+    const int rx = ws->rx, ry = ws->ry;
+    const int j0 = jbase + blockIdx.y * kStrip;
+    if (j0 > f.H - 1) return;
+    const int j1 = min(j0 + kStrip - 1, f.H - 1);
+    const int px = ringx(i, rx, f.Wp), pxp = ringx(i + 1, rx, f.Wp);
+    const int jlo0 = ws->jbox[0][0] - ws->wx0, jlo1 = ws->jbox[0][1] - ws->wy0;
+    const int jhi0 = -ws->jbox[0][2] - ws->wx0, jhi1 = -ws->jbox[0][3] - ws->wy0;
+    const bool colInJ = i >= jlo0 && i <= jhi0;
+    auto row = [&](int j) { return (size_t)ringy(j, ry, f.Hp) * f.Wp; };
+    // prologue: H1(j0-1)
+    T h1p;
+    {
+        const int j = j0 - 1;
+        const size_t rb = row(j);
+        const T hold = f.hz[rb + px];
+        if (j >= 0) {
+            h1p = h_update(f, rb + px, f.cls[rb + px], hold, f.ey[rb + px], f.ey[rb + pxp], f.ex[rb + px],
+                           f.ex[row(j0) + px], c);
+        } else {
+            h1p = hold;
+        }
+    }
+    T exo = f.ex[row(j0) + px];    // Ex old at the current row
+    T exnp = 0, eyp_prev = 0, ey_prev = 0, h1pp = 0;
+    uint8_t clp = 0;
+    size_t qprev = 0;
+    for (int j = j0; j <= j1 + 1; ++j) {
+        const size_t rb = row(j);
+        const size_t q = rb + px;
+        const T exo_next = f.ex[row(j + 1) + px];
+        const T ey = f.ey[q], eyp = f.ey[rb + pxp];
+        const uint8_t cl = f.cls[q];
+        T h1, exn;
+        if (j <= f.H - 1) {
+            h1 = h_update(f, q, cl, f.hz[q], ey, eyp, exo, exo_next, c);
+            const T jv = (colInJ && j >= jlo1 && j <= jhi1) ? f.jx[q] : T(0);
+            exn = ex_update(f, q, cl, exo, h1, h1p, jv, c);
+        } else {
+            h1 = f.hz[q];
+            exn = exo;
+        }
+        if (j > j0) {
+            // H2(j-1) from H1(j-1), Ex new at j-1 and j
+            f.hz2[qprev] = h_update(f, qprev, clp, h1pp, ey_prev, eyp_prev, exnp, exn, c);
+            f.ex2[qprev] = exnp;
+        }
+        h1pp = h1;
+        h1p = h1;
+        exnp = exn;
+        exo = exo_next;
+        ey_prev = ey; eyp_prev = eyp; clp = cl; qprev = q;
+    }
+}
+
+// Ey (TETMSolver.hh:241-249, conformal TETMSolver.cpp:117-127); clears Jy in
+// its footprint after use and Jx in the Jx footprint, leaving both current
+// planes zero for the next deposits.
+template <class T>
+__global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const WinState *ws)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.W) return;
+    const int rx = ws->rx, ry = ws->ry;
+    const int px = ringx(i, rx, f.Wp), pxm = ringx(i - 1, rx, f.Wp);
+    const int jlo0 = ws->jbox[1][0] - ws->wx0, jlo1 = ws->jbox[1][1] - ws->wy0;
+    const int jhi0 = -ws->jbox[1][2] - ws->wx0, jhi1 = -ws->jbox[1][3] - ws->wy0;
+    const bool colInJ = i >= jlo0 && i <= jhi0;
+    const int xlo0 = ws->jbox[0][0] - ws->wx0, xlo1 = ws->jbox[0][1] - ws->wy0;
+    const int xhi0 = -ws->jbox[0][2] - ws->wx0, xhi1 = -ws->jbox[0][3] - ws->wy0;
+    const bool colInJx = i >= xlo0 && i <= xhi0;
+    const int j0 = blockIdx.y * kStrip, j1 = min(j0 + kStrip - 1, f.H - 1);
+    for (int j = j0; j <= j1; ++j) {
+        const size_t rb = (size_t)ringy(j, ry, f.Hp) * f.Wp;
+        const size_t q = rb + px;
+        const uint8_t cl = f.cls[q];
+        const T d = f.hz[q] - f.hz[rb + pxm];
+        T jv = 0;
+        if (colInJ && j >= jlo1 && j <= jhi1) { jv = f.jy[q]; f.jy[q] = 0; }
+        if (colInJx && j >= xlo1 && j <= xhi1) f.jx[q] = 0;
+        if (cl == 1) f.ey[q] = f.ey[q] - (c.sey * d + c.dteps * jv);
+        else if (cl == 2 && f.cut[f.plane + q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea))
+            f.ey[q] = f.ey[q] + c.ce * (-d * c.ihx - jv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// window move decision: keep the bunch's leading edge `lead` cells from the
+// window front and its centre near the window centre in y (at most one cell
+// per axis per step, as moveMesh, main.cpp:562-652).
+__global__ void k_move(WinState *ws, int W, int H, int Wp, int Hp, int lead)
+{
+    const int bmaxx = -ws->bb[2] - ws->wx0;
+    const int bminy = ws->bb[1] - ws->wy0, bmaxy = -ws->bb[3] - ws->wy0;
+    int mi = bmaxx > W - 1 - lead ? 1 : 0;
+    const int cy2 = bminy + bmaxy, mid2 = H - 1;
+    int mj = cy2 > mid2 + 2 ? 1 : (cy2 < mid2 - 2 ? -1 : 0);
+    ws->mi = mi; ws->mj = mj;
+    ws->wx0 += mi; ws->wy0 += mj;
+    ws->rx = (ws->rx + mi) % Wp;
+    ws->ry = (ws->ry + mj + Hp) % Hp;
+    // the footprint of the next deposits: bunch box + stencil margins + one cell of motion
+    for (int c = 0; c < 2; ++c) {
+        ws->jbox[c][0] = ws->bb[0] - 6; ws->jbox[c][1] = ws->bb[1] - 6;
+        ws->jbox[c][2] = ws->bb[2] - 6; ws->jbox[c][3] = ws->bb[3] - 6;
+    }
+    ws->bb[0] = INT_MAX; ws->bb[1] = INT_MAX; ws->bb[2] = INT_MAX; ws->bb[3] = INT_MAX;
+    ws->steps += 1;
+}
+
+// clear + geometry of the recycled guard column (mi = 1) / row (mj = +-1)
+template <class T>
+__global__ void k_fresh(Fields<T> f, Geo g, const WinState *ws)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mi = ws->mi, mj = ws->mj;
+    const int ncol = f.H + 2 * G, nrow = f.W + 2 * G;
+    int i, j;
+    if (t < ncol) {
+        if (mi == 0) return;
+        i = f.W - 1 + G; j = t - G;
+    } else if (t < ncol + nrow) {
+        if (mj == 0) return;
+        i = t - ncol - G; j = mj > 0 ? f.H - 1 + G : -G;
+    } else {
+        return;
+    }
+    const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
+    f.ex[q] = 0; f.ey[q] = 0; f.hz[q] = 0; f.jx[q] = 0; f.jy[q] = 0; f.ex2[q] = 0; f.hz2[q] = 0;
+    cell_geometry(g, f, ws->wx0 + i, ws->wy0 + j, q);
+}
+
+// ---------------------------------------------------------------------------
+// cell sort keys and permutation
+template <class T>
+__global__ void k_sort_keys(Parts<T> p, const WinState *ws, int Wk, unsigned *keys, int *idx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.n) return;
+    const int i = p.gx[k] - ws->wx0 + 8, j = p.gy[k] - ws->wy0 + 8;
+    const unsigned tile = (unsigned)(j >> 2) * (unsigned)(Wk >> 4) + (unsigned)(i >> 4);
+    keys[k] = (tile << 6) | ((unsigned)(j & 3) << 4) | (unsigned)(i & 15);
+    idx[k] = k;
+}
+
+template <class T>
+__global__ void k_permute(Parts<T> src, Parts<T> dst, const int *idx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= src.n) return;
+    const int s = idx[k];
+    dst.gx[k] = src.gx[s]; dst.gy[k] = src.gy[s];
+    dst.fx[k] = src.fx[s]; dst.fy[k] = src.fy[s];
+    dst.px[k] = src.px[s]; dst.py[k] = src.py[s];
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: moments (double-double) and the wake along the centroid line
+struct DD {
+    double hi, lo;
+};
+__device__ inline DD dd_add(DD x, DD y)
+{
+    double s = x.hi + y.hi;
+    double bb = s - x.hi;
+    double e = (x.hi - (s - bb)) + (y.hi - bb);
+    e += x.lo + y.lo;
+    const double hi = s + e;
+    return DD{hi, e - (hi - s)};
+}
+constexpr int kRedBlocks = 240;
+
+template <class T>
+__global__ void k_moments(Parts<T> p, Geo g, const WinState *ws, DD *partial)
+{
+    __shared__ DD sh[4][kB];
+    DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < p.n; k += gridDim.x * blockDim.x) {
+        const double X = g.org[0] + ((double)p.gx[k] + (double)p.fx[k]) * g.h[0];
+        const double Y = g.org[1] + ((double)p.gy[k] + (double)p.fy[k]) * g.h[1];
+        acc[0] = dd_add(acc[0], DD{X, 0});
+        acc[1] = dd_add(acc[1], DD{ws->p0[0] + (double)p.px[k], 0});
+        acc[2] = dd_add(acc[2], DD{Y, 0});
+        acc[3] = dd_add(acc[3], DD{ws->p0[1] + (double)p.py[k], 0});
+    }
+    for (int m = 0; m < 4; ++m) sh[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int m = 0; m < 4; ++m) sh[m][threadIdx.x] = dd_add(sh[m][threadIdx.x], sh[m][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 4; ++m) partial[blockIdx.x * 4 + m] = sh[m][0];
+}
+
+__global__ void k_moments_final(const DD *partial, int n, double *mom)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    for (int m = 0; m < 4; ++m) {
+        DD s{0, 0};
+        for (int b = 0; b < kRedBlocks; ++b) s = dd_add(s, partial[b * 4 + m]);
+        mom[m] = (s.hi + s.lo) / n;
+    }
+}
+
+// Wake: E along the line through the centroid in the direction of the mean
+// momentum, at ns positions spaced h0 centred on the centroid, with the
+// rotation convention of IntegrateElong.cpp:49-61; accumulated with
+// compensation over the sampling steps.
+template <class T>
+__global__ void k_wake(Fields<T> f, Geo g, const WinState *ws, const double *mom, int ns, DD *acc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const double R0 = mom[0], P0 = mom[1], R1 = mom[2], P1 = mom[3];
+    const double lenP = sqrt(P0 * P0 + P1 * P1);
+    const double cosP = P0 / lenP, sinP = P1 / lenP;
+    const double l0 = (double)(k - ns / 2) * g.h[0];
+    const double x = (cosP * l0) + R0;
+    const double y = (-sinP * l0) + R1;
+    // position in global cells
+    const double ux = (x - g.org[0]) / g.h[0], uy = (y - g.org[1]) / g.h[1];
+    const int gx = (int)floor(ux), gy = (int)floor(uy);
+    const int i = gx - ws->wx0, j = gy - ws->wy0;
+    if (i < 1 || i > f.W - 2 || j < 1 || j > f.H - 2) return;
+    const vec2<T> pp = pm::mk((T)(ux - gx), (T)(uy - gy));
+    auto fx = [&](long long I, long long J) {
+        return f.ex[(size_t)ringy(j + (int)J, ws->ry, f.Hp) * f.Wp + ringx(i + (int)I, ws->rx, f.Wp)];
+    };
+    auto fy = [&](long long I, long long J) {
+        return f.ey[(size_t)ringy(j + (int)J, ws->ry, f.Hp) * f.Wp + ringx(i + (int)I, ws->rx, f.Wp)];
+    };
+    const double ex = (double)pm::gather(pm::edge_stencil(0, 0, pp), false, fx);
+    const double ey = (double)pm::gather(pm::edge_stencil(0, 1, pp), false, fy);
+    acc[k] = dd_add(acc[k], DD{cosP * ex + sinP * ey, 0.0});
+}
+
+// Re-centring of the split momentum: p0 moves to the mean momentum (kept a
+// value of T, so p0 + dP is unchanged up to one rounding of dP).
+template <class T>
+__global__ void k_dp_partial(Parts<T> p, DD *partial)
+{
+    __shared__ DD sh[2][kB];
+    DD acc[2] = {{0, 0}, {0, 0}};
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < p.n; k += gridDim.x * blockDim.x) {
+        acc[0] = dd_add(acc[0], DD{(double)p.px[k], 0});
+        acc[1] = dd_add(acc[1], DD{(double)p.py[k], 0});
+    }
+    sh[0][threadIdx.x] = acc[0];
+    sh[1][threadIdx.x] = acc[1];
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int m = 0; m < 2; ++m) sh[m][threadIdx.x] = dd_add(sh[m][threadIdx.x], sh[m][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[blockIdx.x * 2] = sh[0][0]; partial[blockIdx.x * 2 + 1] = sh[1][0]; }
+}
+
+template <class T>
+__global__ void k_dp_final(const DD *partial, int n, WinState *ws, double *shift)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    for (int m = 0; m < 2; ++m) {
+        DD s{0, 0};
+        for (int b = 0; b < kRedBlocks; ++b) s = dd_add(s, partial[b * 2 + m]);
+        const double mean = (s.hi + s.lo) / n;
+        const T pn = (T)(ws->p0[m] + mean);
+        shift[m] = (double)pn - ws->p0[m];
+        ws->p0[m] = (double)pn;
+    }
+}
+
+template <class T>
+__global__ void k_dp_apply(Parts<T> p, const double *shift)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.n) return;
+    p.px[k] = p.px[k] - (T)shift[0];
+    p.py[k] = p.py[k] - (T)shift[1];
+}
+
+__global__ void k_state_init(WinState *ws, int wx0, int wy0)
+{
+    ws->wx0 = wx0; ws->wy0 = wy0; ws->rx = 0; ws->ry = 0; ws->mi = 0; ws->mj = 0;
+    for (int k = 0; k < 4; ++k) ws->bb[k] = INT_MAX;
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < 4; ++k) ws->jbox[c][k] = INT_MAX;
+    ws->err = 0;
+    ws->steps = 0;
+}
+
+__global__ void k_set_p0(WinState *ws, double p0x, double p0y)
+{
+    ws->p0[0] = p0x;
+    ws->p0[1] = p0y;
+}
+
+__global__ void k_bbox_init(Parts<float> pf, Parts<double> pd, bool isf, WinState *ws)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = isf ? pf.n : pd.n;
+    if (k >= n) return;
+    const int gx = isf ? pf.gx[k] : pd.gx[k], gy = isf ? pf.gy[k] : pd.gy[k];
+    atomicMin(&ws->bb[0], gx); atomicMin(&ws->bb[1], gy);
+    atomicMin(&ws->bb[2], -gx); atomicMin(&ws->bb[3], -gy);
+}
+
+}  // namespace mw
+
+// ===========================================================================
+struct csrst_mw {
+    std::string err;
+    csrst_mw_params p{};
+    mw::Geo geo{};
+    double dt = 0;
+    hipStream_t s = nullptr;
+    int W = 0, H = 0, Wp = 0, Hp = 0;
+    size_t plane = 0;
+    bool f32 = true;
+    // storage
+    DBuf<unsigned char> fieldsBuf, partBuf, partBuf2;
+    DBuf<uint8_t> cls;
+    DBuf<mw::WinState> ws;
+    DBuf<unsigned> keys, keys2;
+    DBuf<int> idx, idx2;
+    DBuf<unsigned char> cubTmp;
+    size_t cubBytes = 0;
+    DBuf<mw::DD> partial, wacc;
+    DBuf<double> mom, shift;
+    int n = 0;
+    double q = 0;
+    int nsWake = 0;
+    int steps = 0;
+    int lead = 32;
+    hipEvent_t ev[12];
+    double lastMs[5] = {0, 0, 0, 0, 0};
+    bool ready = false;
+
+    int cur = 0;   // which of the Ex/Hz ping-pong buffers is current
+
+    template <class T>
+    mw::Fields<T> fields()
+    {
+        mw::Fields<T> f;
+        T *b = reinterpret_cast<T *>(fieldsBuf.p);
+        T *exa = b, *exb = b + 5 * plane, *hza = b + 2 * plane, *hzb = b + 6 * plane;
+        f.ex = cur ? exb : exa; f.ex2 = cur ? exa : exb;
+        f.hz = cur ? hzb : hza; f.hz2 = cur ? hza : hzb;
+        f.ey = b + plane; f.jx = b + 3 * plane; f.jy = b + 4 * plane;
+        f.cut = b + 7 * plane;
+        f.cls = cls.p;
+        f.W = W; f.H = H; f.Wp = Wp; f.Hp = Hp; f.plane = plane;
+        return f;
+    }
+    template <class T>
+    mw::Parts<T> parts(DBuf<unsigned char> &buf)
+    {
+        mw::Parts<T> pp;
+        unsigned char *b = buf.p;
+        const size_t nn = (size_t)n;
+        pp.gx = reinterpret_cast<int *>(b);
+        pp.gy = reinterpret_cast<int *>(b + 4 * nn);
+        T *r = reinterpret_cast<T *>(b + 8 * nn);
+        pp.fx = r; pp.fy = r + nn; pp.px = r + 2 * nn; pp.py = r + 3 * nn;
+        pp.n = n;
+        return pp;
+    }
+};
+
+namespace {
+
+thread_local std::string g_mw_err;
+
+int mw_fail(csrst_mw *m, const std::exception &e)
+{
+    const Error *he = dynamic_cast<const Error *>(&e);
+    if (m) m->err = e.what();
+    g_mw_err = e.what();
+    return he ? he->code : CSRST_EINVAL;
+}
+
+template <class T>
+void sweep_consts(const csrst_mw *m, mw::SweepConst<T> &c)
+{
+    const double etax = pm::kC * m->dt / m->geo.h[0];
+    const double etay = pm::kC * m->dt / m->geo.h[1];
+    const double Z0 = sqrt(pm::kMu0 / pm::kEps0);
+    c.sx = (T)(0.5 * etax / Z0);
+    c.sy = (T)(0.5 * etay / Z0);
+    c.sex = (T)(etay * Z0);
+    c.sey = (T)(etax * Z0);
+    c.dteps = (T)(pm::kC * m->dt * Z0);
+    c.cx = (T)(0.5 * (m->dt / (m->geo.h[0] * pm::kMu0)));
+    c.cy = (T)(0.5 * (m->dt / (m->geo.h[1] * pm::kMu0)));
+    c.ce = (T)(m->dt / pm::kEps0);
+    c.ihx = (T)(1.0 / m->geo.h[0]);
+    c.ihy = (T)(1.0 / m->geo.h[1]);
+}
+
+template <class T>
+void push_consts(const csrst_mw *m, mw::PushConst<T> &c)
+{
+    const double half = 0.5 * m->dt * pm::kC;
+    c.hx = (T)(half / m->geo.h[0]);
+    c.hy = (T)(half / m->geo.h[1]);
+    const double f1 = half * (-1e-9 / pm::kMeGeV);
+    c.fact1 = (T)f1;
+    c.fact2 = (T)(f1 * pm::kC);
+    c.q = (T)m->q;
+    c.qodt_x = (T)(m->q / (m->geo.h[0] * m->dt));
+    c.qodt_y = (T)(m->q / (m->geo.h[1] * m->dt));
+    c.odt_x = (T)(1.0 / (m->geo.h[0] * m->dt));
+    c.odt_y = (T)(1.0 / (m->geo.h[1] * m->dt));
+}
+
+template <class T, int ORDER, int COMP>
+void launch_push(csrst_mw *m)
+{
+    mw::PushConst<T> c;
+    push_consts(m, c);
+    hipLaunchKernelGGL((mw::k_push<T, ORDER, COMP>), dim3((m->n + mw::kChunk - 1) / mw::kChunk), dim3(mw::kB), 0, m->s,
+                       m->parts<T>(m->partBuf), m->fields<T>(), m->geo, c, m->ws.p);
+}
+
+template <class T, int COMP>
+void push_dispatch(csrst_mw *m)
+{
+    if (m->p.order == 0) launch_push<T, 0, COMP>(m);
+    else if (m->p.order == 1) launch_push<T, 1, COMP>(m);
+    else launch_push<T, 2, COMP>(m);
+    HU_LAUNCH_CHECK();
+}
+
+template <class T>
+void sort_particles(csrst_mw *m)
+{
+    const int nb = (m->n + mw::kB - 1) / mw::kB;
+    const int Wk = (m->W + 16 + 15) & ~15;
+    mw::k_sort_keys<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->ws.p, Wk, m->keys.p, m->idx.p);
+    HU_LAUNCH_CHECK();
+    HU_CHECK(hipcub::DeviceRadixSort::SortPairs(m->cubTmp.p, m->cubBytes, m->keys.p, m->keys2.p, m->idx.p, m->idx2.p,
+                                                m->n, 0, 32, m->s));
+    mw::k_permute<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->parts<T>(m->partBuf2), m->idx2.p);
+    HU_LAUNCH_CHECK();
+    std::swap(m->partBuf.p, m->partBuf2.p);
+    std::swap(m->partBuf.n, m->partBuf2.n);
+    mw::k_dp_partial<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->partial.p);
+    mw::k_dp_final<T><<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->ws.p, m->shift.p);
+    mw::k_dp_apply<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->shift.p);
+    HU_LAUNCH_CHECK();
+}
+
+template <class T>
+void wake(csrst_mw *m)
+{
+    mw::k_moments<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->geo, m->ws.p, m->partial.p);
+    mw::k_moments_final<<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
+    mw::k_wake<T><<<(m->nsWake + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(m->fields<T>(), m->geo, m->ws.p, m->mom.p,
+                                                                          m->nsWake, m->wacc.p);
+    HU_LAUNCH_CHECK();
+}
+
+template <class T>
+void one_step(csrst_mw *m, bool timed)
+{
+    mw::SweepConst<T> sc;
+    sweep_consts(m, sc);
+    const mw::Fields<T> f = m->fields<T>();
+    const dim3 gs((m->W + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
+    if (m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0) {
+        if (timed) HU_CHECK(hipEventRecord(m->ev[6], m->s));
+        sort_particles<T>(m);
+        if (timed) HU_CHECK(hipEventRecord(m->ev[7], m->s));
+    }
+    if (timed) HU_CHECK(hipEventRecord(m->ev[0], m->s));
+    if (m->n > 0) push_dispatch<T, 0>(m);
+    if (timed) HU_CHECK(hipEventRecord(m->ev[1], m->s));
+    hipLaunchKernelGGL((mw::k_heh<T>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0);
+    HU_LAUNCH_CHECK();
+    m->cur ^= 1;
+    const mw::Fields<T> f2 = m->fields<T>();
+    if (timed) HU_CHECK(hipEventRecord(m->ev[2], m->s));
+    if (m->n > 0) push_dispatch<T, 1>(m);
+    if (timed) HU_CHECK(hipEventRecord(m->ev[3], m->s));
+    hipLaunchKernelGGL((mw::k_ey<T>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p);
+    HU_LAUNCH_CHECK();
+    if (m->n > 0 && m->p.statModulus > 0 && m->steps % m->p.statModulus == m->p.statModulus - 1) wake<T>(m);
+    if (timed) HU_CHECK(hipEventRecord(m->ev[4], m->s));
+    if (m->n > 0) {
+        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, m->lead);
+        const int nf = m->H + 2 * mw::G + m->W + 2 * mw::G;
+        hipLaunchKernelGGL((mw::k_fresh<T>), dim3((nf + mw::kB - 1) / mw::kB), dim3(mw::kB), 0, m->s, f2, m->geo, m->ws.p);
+        HU_LAUNCH_CHECK();
+    }
+    if (timed) HU_CHECK(hipEventRecord(m->ev[5], m->s));
+    ++m->steps;
+}
+
+void check_state(csrst_mw *m)
+{
+    mw::WinState h;
+    HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
+    HU_CHECK(hipStreamSynchronize(m->s));
+    if (h.err & 1) throw Error(CSRST_EWINDOW, "particle stencil left the window");
+    if (h.err & 2) throw Error(CSRST_EWINDOW, "particle crossed too many cell lines in one push");
+}
+
+}  // namespace
+
+extern "C" {
+
+int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
+{
+    std::unique_ptr<csrst_mw> m(new csrst_mw());
+    try {
+        if (!p || !out) throw Error(CSRST_EINVAL, "null argument");
+        if (p->W < 16 || p->H < 16) throw Error(CSRST_EINVAL, "window too small");
+        if (p->order < 0 || p->order > 2) throw Error(CSRST_EINVAL, "order must be 0..2");
+        if (p->precision != 0 && p->precision != 1) throw Error(CSRST_EINVAL, "precision must be 0 or 1");
+        if (!(p->dx > 0) || !(p->dy > 0)) throw Error(CSRST_EINVAL, "spacing must be positive");
+        if (p->dx > p->dy * (1 + 1e-12)) throw Error(CSRST_EINVAL, "TE/TM scheme needs dx <= dy");
+        m->p = *p;
+        m->f32 = p->precision == 1;
+        HU_CHECK(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
+        for (auto &e : m->ev) HU_CHECK(hipEventCreate(&e));
+        // geometry: the bend of Bend.cpp with the configured spacing; the
+        // virtual global mesh origin as in Bend::Init (vertex 1 on x = 0)
+        geom::Bend &b = m->geo.bend;
+        geom::bend_init(b, p->lenBefore, p->lenAfter, p->width, p->phiDeg, p->ekinMeV, p->bz, 1000, 1000);
+        const double dwid0 = b.dwid - b.hr[1];
+        m->geo.h[0] = p->dx;
+        m->geo.h[1] = p->dy;
+        m->geo.org[0] = -p->dx;
+        m->geo.org[1] = b.bz <= 0.0 ? (p->width - dwid0) - p->dy : -p->dy;
+        b.hr[0] = p->dx; b.hr[1] = p->dy;
+        b.origin[0] = m->geo.org[0]; b.origin[1] = m->geo.org[1];
+        const double cfl = p->cfl > 0 ? p->cfl : 1.0;
+        m->dt = cfl * p->dx / pm::kC;
+        m->W = p->W; m->H = p->H;
+        m->Wp = p->W + 2 * mw::G; m->Hp = p->H + 2 * mw::G;
+        m->plane = (size_t)m->Wp * m->Hp;
+        const size_t rs = m->f32 ? 4 : 8;
+        m->fieldsBuf.alloc(m->plane * 12 * rs);   // ex ey hz jx jy ex' hz' + 5 cut planes
+        m->fieldsBuf.zero(m->s);
+        m->cls.alloc(m->plane);
+        m->cls.zero(m->s);
+        m->ws.alloc(1);
+        m->partial.alloc((size_t)mw::kRedBlocks * 4);
+        m->mom.alloc(4);
+        m->shift.alloc(2);
+        HU_CHECK(hipStreamSynchronize(m->s));
+        *out = m.release();
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(nullptr, e);
+    }
+}
+
+int csrst_mw_destroy(csrst_mw *m)
+{
+    if (!m) return CSRST_OK;
+    if (m->s) { (void)hipStreamSynchronize(m->s); (void)hipStreamDestroy(m->s); }
+    for (auto &e : m->ev) if (e) (void)hipEventDestroy(e);
+    delete m;
+    return CSRST_OK;
+}
+
+int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const double *Q, int n)
+{
+    try {
+        if (!m || !R || !P || !Q || n <= 0) throw Error(CSRST_EINVAL, "bad bunch arguments");
+        m->n = n;
+        m->q = Q[0];
+        for (int k = 1; k < n; ++k)
+            if (Q[k] != m->q) throw Error(CSRST_EINVAL, "the moving-window step takes a uniform macro-particle charge");
+        // split positions; window: bunch front `lead` cells behind the window
+        // front, bunch centred in y
+        std::vector<int> gx(n), gy(n);
+        const bool f32 = m->f32;
+        const size_t rs = f32 ? 4 : 8;
+        std::vector<unsigned char> host((size_t)n * (8 + 4 * rs));
+        int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
+        long double sp[2] = {0, 0};
+        for (int k = 0; k < n; ++k) { sp[0] += P[2 * k]; sp[1] += P[2 * k + 1]; }
+        double p0[2] = {(double)(sp[0] / n), (double)(sp[1] / n)};
+        if (f32) { p0[0] = (double)(float)p0[0]; p0[1] = (double)(float)p0[1]; }
+        for (int k = 0; k < n; ++k) {
+            const double ux = (R[2 * k] - m->geo.org[0]) / m->geo.h[0];
+            const double uy = (R[2 * k + 1] - m->geo.org[1]) / m->geo.h[1];
+            const double fx = floor(ux), fy = floor(uy);
+            gx[k] = (int)fx; gy[k] = (int)fy;
+            mnx = std::min(mnx, gx[k]); mxx = std::max(mxx, gx[k]);
+            mny = std::min(mny, gy[k]); mxy = std::max(mxy, gy[k]);
+            memcpy(&host[4 * (size_t)k], &gx[k], 4);
+            memcpy(&host[4 * (size_t)n + 4 * (size_t)k], &gy[k], 4);
+            const double v[4] = {ux - fx, uy - fy, P[2 * k] - p0[0], P[2 * k + 1] - p0[1]};
+            for (int c = 0; c < 4; ++c) {
+                unsigned char *dst = &host[8 * (size_t)n + ((size_t)c * n + k) * rs];
+                if (f32) {
+                    float fv = (float)v[c];
+                    if (c < 2 && fv >= 1.0f) fv = nextafterf(1.0f, 0.0f);
+                    memcpy(dst, &fv, 4);
+                } else {
+                    memcpy(dst, &v[c], 8);
+                }
+            }
+        }
+        if (mxx - mnx + 2 * m->lead >= m->W || mxy - mny + 16 >= m->H)
+            throw Error(CSRST_EWINDOW, hu::fmt("bunch (%d x %d cells) does not fit the window", mxx - mnx + 1, mxy - mny + 1));
+        const int wx0 = mxx - (m->W - 1 - m->lead);
+        const int wy0 = (mny + mxy) / 2 - (m->H - 1) / 2;
+        m->partBuf.alloc(host.size());
+        m->partBuf2.alloc(host.size());
+        HU_CHECK(hipMemcpy(m->partBuf.p, host.data(), host.size(), hipMemcpyHostToDevice));
+        mw::k_state_init<<<1, 1, 0, m->s>>>(m->ws.p, wx0, wy0);
+        mw::k_set_p0<<<1, 1, 0, m->s>>>(m->ws.p, p0[0], p0[1]);
+        const dim3 gg((m->W + 2 * mw::G + mw::kB - 1) / mw::kB, m->H + 2 * mw::G);
+        if (f32) hipLaunchKernelGGL((mw::k_geom_all<float>), gg, dim3(mw::kB), 0, m->s, m->fields<float>(), m->geo, m->ws.p);
+        else hipLaunchKernelGGL((mw::k_geom_all<double>), gg, dim3(mw::kB), 0, m->s, m->fields<double>(), m->geo, m->ws.p);
+        HU_LAUNCH_CHECK();
+        mw::Parts<float> pf{}; mw::Parts<double> pd{};
+        if (f32) pf = m->parts<float>(m->partBuf); else pd = m->parts<double>(m->partBuf);
+        mw::k_bbox_init<<<(n + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(pf, pd, f32, m->ws.p);
+        // initial footprint = particle box +- 4 (k_move convention)
+        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, 1 << 30);
+        HU_LAUNCH_CHECK();
+        // k_move with a huge lead makes no move; reset the move counters
+        mw::WinState h;
+        HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
+        HU_CHECK(hipStreamSynchronize(m->s));
+        h.steps = 0; h.mi = 0; h.mj = 0;
+        for (int c = 0; c < 4; ++c) h.bb[c] = INT_MAX;
+        HU_CHECK(hipMemcpy(m->ws.p, &h, sizeof(h), hipMemcpyHostToDevice));
+        // sort scratch
+        m->keys.alloc(n); m->keys2.alloc(n); m->idx.alloc(n); m->idx2.alloc(n);
+        m->cubBytes = 0;
+        HU_CHECK(hipcub::DeviceRadixSort::SortPairs((void *)nullptr, m->cubBytes, m->keys.p, m->keys2.p, m->idx.p,
+                                                    m->idx2.p, n, 0, 32, m->s));
+        m->cubTmp.alloc(m->cubBytes + 16);
+        m->nsWake = m->W - 4;
+        m->wacc.alloc(m->nsWake);
+        m->wacc.zero(m->s);
+        m->steps = 0;
+        m->ready = true;
+        if (m->p.sortEvery > 0) {
+            if (f32) sort_particles<float>(m); else sort_particles<double>(m);
+        }
+        HU_CHECK(hipStreamSynchronize(m->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_step(csrst_mw *m, int nsteps)
+{
+    try {
+        if (!m || !m->ready) throw Error(CSRST_ESTATE, "csrst_mw_step before csrst_mw_init_bunch");
+        double acc[5] = {0, 0, 0, 0, 0};
+        int nsorted = 0;
+        for (int k = 0; k < nsteps; ++k) {
+            const bool sorted = m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0;
+            if (m->f32) one_step<float>(m, true); else one_step<double>(m, true);
+            HU_CHECK(hipEventSynchronize(m->ev[5]));
+            float t01, t12, t23, t34, t45, t67 = 0;
+            HU_CHECK(hipEventElapsedTime(&t01, m->ev[0], m->ev[1]));
+            HU_CHECK(hipEventElapsedTime(&t12, m->ev[1], m->ev[2]));
+            HU_CHECK(hipEventElapsedTime(&t23, m->ev[2], m->ev[3]));
+            HU_CHECK(hipEventElapsedTime(&t34, m->ev[3], m->ev[4]));
+            HU_CHECK(hipEventElapsedTime(&t45, m->ev[4], m->ev[5]));
+            if (sorted) { HU_CHECK(hipEventElapsedTime(&t67, m->ev[6], m->ev[7])); ++nsorted; }
+            acc[0] += t01 + t23;
+            acc[1] += t12 + t34;
+            acc[2] += t45;
+            acc[3] += t67;
+        }
+        for (int k = 0; k < 4; ++k) m->lastMs[k] = nsteps ? acc[k] / nsteps : 0;
+        check_state(m);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_place(csrst_mw *m, int wx0, int wy0)
+{
+    try {
+        if (!m) throw Error(CSRST_EINVAL, "null");
+        m->n = 0;
+        m->cur = 0;
+        mw::k_state_init<<<1, 1, 0, m->s>>>(m->ws.p, wx0, wy0);
+        mw::k_set_p0<<<1, 1, 0, m->s>>>(m->ws.p, 0.0, 0.0);
+        const dim3 gg((m->W + 2 * mw::G + mw::kB - 1) / mw::kB, m->H + 2 * mw::G);
+        if (m->f32) hipLaunchKernelGGL((mw::k_geom_all<float>), gg, dim3(mw::kB), 0, m->s, m->fields<float>(), m->geo, m->ws.p);
+        else hipLaunchKernelGGL((mw::k_geom_all<double>), gg, dim3(mw::kB), 0, m->s, m->fields<double>(), m->geo, m->ws.p);
+        HU_LAUNCH_CHECK();
+        m->nsWake = m->W - 4;
+        m->wacc.alloc(m->nsWake);
+        m->wacc.zero(m->s);
+        m->steps = 0;
+        m->ready = true;
+        HU_CHECK(hipStreamSynchronize(m->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_upload_fields(csrst_mw *m, const double *in, int count)
+{
+    try {
+        if (!m || !in || !m->ready) throw Error(CSRST_ESTATE, "not initialised");
+        if ((size_t)count != 6 * m->plane) throw Error(CSRST_EINVAL, "size mismatch");
+        HU_CHECK(hipStreamSynchronize(m->s));
+        mw::WinState h;
+        HU_CHECK(hipMemcpy(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost));
+        const size_t rs = m->f32 ? 4 : 8;
+        std::vector<unsigned char> raw(7 * m->plane * rs);
+        HU_CHECK(hipMemcpy(raw.data(), m->fieldsBuf.p, raw.size(), hipMemcpyDeviceToHost));
+        const int exPlane = m->cur ? 5 : 0, hzPlane = m->cur ? 6 : 2;
+        const int dst[5] = {exPlane, 1, hzPlane, 3, 4};
+        for (int pl = 0; pl < 5; ++pl)
+            for (int j = -mw::G; j < m->H + mw::G; ++j)
+                for (int i = -mw::G; i < m->W + mw::G; ++i) {
+                    int px = (i + mw::G + h.rx) % m->Wp, py = (j + mw::G + h.ry) % m->Hp;
+                    if (px < 0) px += m->Wp;
+                    if (py < 0) py += m->Hp;
+                    const size_t q = (size_t)py * m->Wp + px;
+                    const size_t lq = (size_t)(j + mw::G) * m->Wp + (i + mw::G);
+                    double v = pl < 2 ? in[(size_t)pl * m->plane + lq]
+                             : pl == 2 ? in[2 * m->plane + lq] + in[3 * m->plane + lq]
+                                       : in[(size_t)(pl + 1) * m->plane + lq];
+                    unsigned char *d = &raw[((size_t)dst[pl] * m->plane + q) * rs];
+                    if (m->f32) { float fv = (float)v; memcpy(d, &fv, 4); }
+                    else memcpy(d, &v, 8);
+                }
+        HU_CHECK(hipMemcpy(m->fieldsBuf.p, raw.data(), raw.size(), hipMemcpyHostToDevice));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_timing(csrst_mw *m, double *ms, int cap)
+{
+    if (!m || !ms) return CSRST_EINVAL;
+    for (int k = 0; k < cap && k < 5; ++k) ms[k] = m->lastMs[k];
+    return CSRST_OK;
+}
+
+int csrst_mw_download_fields(csrst_mw *m, double *out, int cap, int *count)
+{
+    try {
+        if (!m) throw Error(CSRST_EINVAL, "null");
+        const size_t cnt = 6 * m->plane;
+        if (count) *count = (int)cnt;
+        if (!out) return CSRST_OK;
+        if ((size_t)cap < cnt) throw Error(CSRST_EINVAL, "capacity too small");
+        HU_CHECK(hipStreamSynchronize(m->s));
+        mw::WinState h;
+        HU_CHECK(hipMemcpy(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost));
+        const size_t rs = m->f32 ? 4 : 8;
+        std::vector<unsigned char> raw(7 * m->plane * rs);
+        HU_CHECK(hipMemcpy(raw.data(), m->fieldsBuf.p, raw.size(), hipMemcpyDeviceToHost));
+        const int exPlane = m->cur ? 5 : 0, hzPlane = m->cur ? 6 : 2;
+        // unroll the ring into logical order; planes ex ey hz/2 hz/2 jx jy
+        for (int pl = 0; pl < 6; ++pl) {
+            const int src = pl == 0 ? exPlane : pl == 1 ? 1 : (pl < 4 ? hzPlane : pl - 1);
+            for (int j = -mw::G; j < m->H + mw::G; ++j)
+                for (int i = -mw::G; i < m->W + mw::G; ++i) {
+                    int px = (i + mw::G + h.rx) % m->Wp, py = (j + mw::G + h.ry) % m->Hp;
+                    if (px < 0) px += m->Wp;
+                    if (py < 0) py += m->Hp;
+                    const size_t q = (size_t)py * m->Wp + px;
+                    double v;
+                    if (m->f32) { float fv; memcpy(&fv, &raw[((size_t)src * m->plane + q) * 4], 4); v = fv; }
+                    else memcpy(&v, &raw[((size_t)src * m->plane + q) * 8], 8);
+                    if (pl == 2 || pl == 3) v *= 0.5;
+                    out[(size_t)pl * m->plane + (size_t)(j + mw::G) * m->Wp + (i + mw::G)] = v;
+                }
+        }
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_download_particles(csrst_mw *m, double *R, double *P, int n)
+{
+    try {
+        if (!m || n != m->n) throw Error(CSRST_EINVAL, "particle count mismatch");
+        HU_CHECK(hipStreamSynchronize(m->s));
+        const size_t rs = m->f32 ? 4 : 8;
+        std::vector<unsigned char> host((size_t)n * (8 + 4 * rs));
+        HU_CHECK(hipMemcpy(host.data(), m->partBuf.p, host.size(), hipMemcpyDeviceToHost));
+        mw::WinState hs;
+        HU_CHECK(hipMemcpy(&hs, m->ws.p, sizeof(hs), hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) {
+            int gx, gy;
+            memcpy(&gx, &host[4 * (size_t)k], 4);
+            memcpy(&gy, &host[4 * (size_t)n + 4 * (size_t)k], 4);
+            double v[4];
+            for (int c = 0; c < 4; ++c) {
+                const unsigned char *src = &host[8 * (size_t)n + ((size_t)c * n + k) * rs];
+                if (m->f32) { float fv; memcpy(&fv, src, 4); v[c] = fv; }
+                else memcpy(&v[c], src, 8);
+            }
+            if (R) {
+                R[2 * k] = m->geo.org[0] + ((double)gx + v[0]) * m->geo.h[0];
+                R[2 * k + 1] = m->geo.org[1] + ((double)gy + v[1]) * m->geo.h[1];
+            }
+            if (P) { P[2 * k] = hs.p0[0] + v[2]; P[2 * k + 1] = hs.p0[1] + v[3]; }
+        }
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_info(csrst_mw *m, double *info)
+{
+    try {
+        if (!m || !info) throw Error(CSRST_EINVAL, "null");
+        mw::WinState h;
+        HU_CHECK(hipStreamSynchronize(m->s));
+        HU_CHECK(hipMemcpy(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost));
+        info[0] = m->dt;
+        info[1] = m->geo.org[0] + h.wx0 * m->geo.h[0];
+        info[2] = m->geo.org[1] + h.wy0 * m->geo.h[1];
+        info[3] = h.wx0; info[4] = h.wy0; info[5] = m->W; info[6] = m->H;
+        info[7] = m->steps; info[8] = m->nsWake; info[9] = m->n;
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_wake(csrst_mw *m, double *out, int cap, int *count)
+{
+    try {
+        if (!m || !m->ready) throw Error(CSRST_ESTATE, "not initialised");
+        if (count) *count = m->nsWake;
+        if (!out) return CSRST_OK;
+        if (cap < m->nsWake) throw Error(CSRST_EINVAL, "capacity too small");
+        std::vector<mw::DD> h(m->nsWake);
+        HU_CHECK(hipStreamSynchronize(m->s));
+        HU_CHECK(hipMemcpy(h.data(), m->wacc.p, h.size() * sizeof(mw::DD), hipMemcpyDeviceToHost));
+        for (int k = 0; k < m->nsWake; ++k) out[k] = h[k].hi + h[k].lo;
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+}  // extern "C"

@@ -273,6 +273,26 @@ PM_HD T gather(const Stencil<T> &s, bool s2bug, F f)
     return acc;
 }
 
+// Same arithmetic with the stencil size fixed at compile time (N = order + 2),
+// so the weights stay in registers.
+template <int N, class T, class F>
+PM_HD T gather_n(const Stencil<T> &s, bool s2bug, F f)
+{
+    T acc = T(0);
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+        T row = s.wx[0] * f(0, b);
+#pragma unroll
+        for (int a = 1; a < N; ++a) {
+            const int aa = (s2bug && N == 4 && b == 3 && a == 3) ? 2 : a;
+            row = row + s.wx[a] * f(aa, b);
+        }
+        const T t = s.wy[b] * row;
+        acc = (b == 0) ? t : acc + t;
+    }
+    return acc;
+}
+
 // Scatter weight of node (a, b) as the reference forms it: (wx*wy)*v for
 // S1/S2, for CIC (wx*wy)*v with the same pairing.
 template <class T>

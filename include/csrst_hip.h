@@ -115,6 +115,13 @@ int csrst_mw_destroy(csrst_mw *m);
 /* Positions R in metres (global chamber frame), P = beta*gamma, Q [C]. */
 int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const double *Q, int n);
 int csrst_mw_step(csrst_mw *m, int nsteps);
+/* Field-only start: window logical cell (0, 0) at global cell (wx0, wy0),
+ * geometry computed, fields zero, no particles (steps then run the field
+ * sweeps alone and the window stays in place). */
+int csrst_mw_place(csrst_mw *m, int wx0, int wy0);
+/* planar [6][H+2][W+2] doubles as csrst_mw_download_fields; the two Hz
+ * planes are summed (the step keeps Hz = Hzx + Hzy). */
+int csrst_mw_upload_fields(csrst_mw *m, const double *in, int count);
 /* planar [6][H+2][W+2] doubles: Ex, Ey, Hzx, Hzy, Jx, Jy (guards incl.) */
 int csrst_mw_download_fields(csrst_mw *m, double *out, int cap, int *count);
 int csrst_mw_download_particles(csrst_mw *m, double *R, double *P, int n);

@@ -1584,6 +1584,59 @@ struct Sim {
         }
     }
 
+    /* the global-grid part of one step (TETMSolver.hh:142-249 without the
+     * window): used to check field sweeps on their own */
+    void field_step()
+    {
+        h_half_global();
+        sc_hz();
+        e_x_global();
+        sc_ex();
+        h_half_global();
+        sc_hz();
+        e_y_global();
+        sc_ey();
+    }
+    void h_half_global()
+    {
+        const double etax = C_LIGHT * dt / hr[0];
+        const double etay = C_LIGHT * dt / hr[1];
+        const double Z0 = std::sqrt(MU_0 / EPS_0);
+        const double sx = 0.5 * etax / Z0, sy = 0.5 * etay / Z0;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) {
+                if (!geo->mask[(size_t)j * p.Nx + i]) continue;
+                H.at(0, i, j) = H.at(0, i, j) - sx * (E.at(1, i + 1, j) - E.at(1, i, j));
+                H.at(1, i, j) = H.at(1, i, j) + sy * (E.at(0, i, j + 1) - E.at(0, i, j));
+            }
+    }
+    void e_x_global()
+    {
+        const double etay = C_LIGHT * dt / hr[1];
+        const double Z0 = std::sqrt(MU_0 / EPS_0);
+        const double dteps = C_LIGHT * dt * Z0;
+        const double s = etay * Z0;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) {
+                if (!geo->mask[(size_t)j * p.Nx + i]) continue;
+                E.at(0, i, j) = E.at(0, i, j) +
+                    (s * (((H.at(0, i, j) - H.at(0, i, j - 1)) + H.at(1, i, j)) - H.at(1, i, j - 1)) - dteps * J.at(0, i, j));
+            }
+    }
+    void e_y_global()
+    {
+        const double etax = C_LIGHT * dt / hr[0];
+        const double Z0 = std::sqrt(MU_0 / EPS_0);
+        const double dteps = C_LIGHT * dt * Z0;
+        const double s = etax * Z0;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) {
+                if (!geo->mask[(size_t)j * p.Nx + i]) continue;
+                E.at(1, i, j) = E.at(1, i, j) -
+                    (s * (((H.at(0, i, j) - H.at(0, i - 1, j)) + H.at(1, i, j)) - H.at(1, i - 1, j)) + dteps * J.at(1, i, j));
+            }
+    }
+
     /* main.cpp:390-506, one iteration */
     void step()
     {
@@ -1828,6 +1881,17 @@ int orc_sim_set_field(void *sp, int which, const double *in)
     }
     return (int)(2 * sz);
 }
+int orc_sim_field_steps(void *sp, int nsteps)
+{
+    try {
+        for (int k = 0; k < nsteps; ++k) as<Sim>(sp)->field_step();
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle field steps: %s\n", e.what());
+        return -1;
+    }
+}
+
 void orc_sim_get_particles(void *sp, double *R, double *P, double *oldR)
 {
     Sim *s = as<Sim>(sp);

@@ -93,6 +93,9 @@ int   orc_sim_get_wake(void *s, double *out);
 void  orc_sim_diag(void *s, double *out);
 /* set a field (same layout as get) */
 int   orc_sim_set_field(void *s, int which, const double *in);
+/* Global-grid sweeps only (no particles, no window): per step
+ * H/2, SC Hz, Ex, SC Ex, H/2, SC Hz, Ey, SC Ey with J as set. */
+int   orc_sim_field_steps(void *s, int nsteps);
 
 #ifdef __cplusplus
 }

@@ -234,6 +234,11 @@ class Sim:
         a = np.ascontiguousarray(np.asarray(arr, np.float64).transpose(1, 2, 0))
         lib().orc_sim_set_field(C.c_void_p(self.h), C.c_int(code), _ptr(a))
 
+    def field_steps(self, n=1):
+        rc = lib().orc_sim_field_steps(C.c_void_p(self.h), C.c_int(n))
+        if rc != 0:
+            raise RuntimeError("oracle field steps failed")
+
     def particles(self):
         R = np.zeros((self.n, 2))
         P = np.zeros((self.n, 2))

@@ -1,0 +1,156 @@
+"""Production moving-window step on the GPU.
+
+- field sweeps (fused H/2-Ex-H/2 row march on the ring buffer, conformal
+  cut cells, merged Hz) against the CPU oracle's global-grid sweeps;
+- FP32 split precision against FP64 on a moving bunch (fields and wake);
+- window motion keeps the bunch inside and positions advance at ~c.
+"""
+import numpy as np
+import pytest
+
+from csrst_amd.mw import MovingWindowStep
+from csrst_amd import gaussian_bunch
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GEOM = dict(lenBefore=0.05, lenAfter=0.01, width=0.0128, phiDeg=10.0, ekinMeV=50.59, bz=0.1704)
+NX, NY = 1100, 120
+
+
+def seeded_fields(o, seed=5):
+    """Smooth random E/H inside the chamber, zero outside."""
+    rng = np.random.default_rng(seed)
+    b = O.Bend(Nx=NX, Ny=NY, **GEOM)
+    mask = b.mask().astype(bool)
+    E = np.zeros((2, NY + 2, NX + 2))
+    H = np.zeros((2, NY + 2, NX + 2))
+    yy, xx = np.mgrid[0:NY, 0:NX]
+    for arr, k in ((E, 0), (E, 1), (H, 0)):
+        f = np.zeros((NY, NX))
+        for _ in range(4):
+            kx, ky = rng.uniform(0.02, 0.2, 2)
+            f += rng.normal() * np.sin(kx * xx + rng.uniform(0, 6)) * np.cos(ky * yy + rng.uniform(0, 6))
+        arr[k, 1:-1, 1:-1] = np.where(mask, f * 1e5, 0.0)
+    H[0, 1:-1, 1:-1] *= 1e-3
+    H[1, 1:-1, 1:-1] = H[0, 1:-1, 1:-1]
+    return E, H
+
+
+@pytest.mark.parametrize("fp32,tol", [(False, 1e-12), (True, 3e-5)])
+def test_field_sweeps_match_oracle(gpu, fp32, tol):
+    o = O.Sim(NX, NY, order=0, statModulus=0, **GEOM)
+    b = O.Bend(Nx=NX, Ny=NY, **GEOM)
+    E, H = seeded_fields(o)
+    o.set_field("E", E)
+    o.set_field("H", H)
+    m = MovingWindowStep(NX, NY, b.hr[0], b.hr[1], order=0, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
+    m.place(0, 0)
+    F = np.zeros((6, NY + 2, NX + 2))
+    F[0], F[1], F[2], F[3] = E[0], E[1], H[0], H[1]
+    m.set_fields(F)
+    for k in range(1, 21):
+        o.field_steps(1)
+        m.step(1)
+        if k in (1, 5, 20):
+            g = m.fields()
+            Eo, Ho = o.field("E"), o.field("H")
+            scaleE = np.abs(Eo).max()
+            scaleH = np.abs(Ho[0] + Ho[1]).max()
+            dE = max(np.abs(g[0] - Eo[0]).max(), np.abs(g[1] - Eo[1]).max()) / scaleE
+            dH = np.abs((g[2] + g[3]) - (Ho[0] + Ho[1])).max() / scaleH
+            assert dE < tol * k and dH < tol * k, "step %d: rel err E %.3e H %.3e" % (k, dE, dH)
+
+
+def run_bunch(fp32, steps, n=200000, order=2, seed=11):
+    R, P, Q = gaussian_bunch(n, (0.03, 0.0064), 0.4e-3, 0.3e-3, 100.0, 1.0, seed=seed, energy_spread=1e-3)
+    b = O.Bend(Nx=NX, Ny=NY, **GEOM)
+    m = MovingWindowStep(512, 100, b.hr[0], b.hr[1], order=order, fp32=fp32, sortEvery=10, statModulus=1, **GEOM)
+    m.init_bunch(R, P, Q)
+    m.pos0 = m.info()["pos"]
+    m.step(steps)
+    return m
+
+
+def test_fp32_split_tracks_fp64(gpu):
+    steps = 60
+    m64 = run_bunch(False, steps)
+    m32 = run_bunch(True, steps)
+    i64, i32 = m64.info(), m32.info()
+    assert i64["pos"] == i32["pos"], "window moves diverged"
+    assert i64["pos"][0] - m64.pos0[0] > steps // 2, "window did not follow the bunch"
+    w64, w32 = m64.wake(), m32.wake()
+    err = np.abs(w32 - w64).max() / np.abs(w64).max()
+    assert err < 1e-4, "FP32 wake rel. error %.3e" % err
+    f64, f32 = m64.fields(), m32.fields()
+    scale = np.abs(f64[0:2]).max()
+    assert np.abs(f32[0:2] - f64[0:2]).max() / scale < 1e-4
+    # particle order follows the cell sort of each run, so compare the
+    # distributions: centroid, rms size and mean momentum
+    R64, P64 = m64.particles()
+    R32, P32 = m32.particles()
+    h = b_h()
+    assert np.abs(R32.mean(0) - R64.mean(0)).max() < 1e-4 * h
+    assert np.abs(R32.std(0) - R64.std(0)).max() < 1e-3 * h
+    assert np.abs(P32.mean(0) - P64.mean(0)).max() < 1e-6 * np.abs(P64.mean(0)).max()
+    xs32, xs64 = np.sort(R32[:, 0]), np.sort(R64[:, 0])
+    assert np.abs(xs32 - xs64).max() < 1e-2 * h
+
+
+def b_h():
+    return O.Bend(Nx=NX, Ny=NY, **GEOM).hr[0]
+
+
+def test_bunch_advances_at_c(gpu):
+    m = run_bunch(True, 30, n=20000)
+    R, P = m.particles()
+    info = m.info()
+    dt = info["dt"]
+    # the bunch centroid moved ~ 30 c dt along the beam direction
+    R0, _, _ = gaussian_bunch(20000, (0.03, 0.0064), 0.4e-3, 0.3e-3, 100.0, 1.0, seed=11, energy_spread=1e-3)
+    adv = np.mean(R[:, 0] - R0[:, 0])
+    assert abs(adv / (30 * 299792458.0 * dt) - 1.0) < 1e-3
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("fp32,tol", [(False, 1e-9), (True, 2e-4)])
+def test_deposit_conserves_charge(gpu, order, fp32, tol):
+    """Gauss law drift: with the bunch moving along x only (no self force at
+    this charge, no bend field before the bend), the deposited Jx keeps
+    div E - rho/eps0 unchanged from step to step (charge-conserving
+    deposit, FirstOrderShapeFunction / SecondOrderShapeFunction continuity;
+    rho with the matching vertex shape of the oracle)."""
+    b = O.Bend(Nx=NX, Ny=NY, **GEOM)
+    hx, hy = b.hr
+    n = 4000
+    rng = np.random.default_rng(3)
+    R = np.stack([0.02 + rng.normal(0, 0.3e-3, n), 0.0064 + rng.normal(0, 0.4e-3, n)], 1)
+    P = np.stack([np.full(n, 99.995), np.zeros(n)], 1)
+    Q = np.full(n, -1e-22)
+    m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
+    m.init_bunch(R, P, Q)
+    eps0 = 8.854187817e-12
+
+    def residual():
+        inf = m.info()
+        wx0, wy0 = inf["pos"]
+        F = m.fields()
+        Ex, Ey = F[0], F[1]      # [H+2, W+2], logical index + 1
+        Rp, _ = m.particles()
+        org = (b.origin[0], b.origin[1])
+        W, H = m.W, m.H
+        rho = O.charge_density(order, (wx0 - 2, wy0 - 2, W + 4, H + 4), org, (hx, hy), Rp, np.full(n, Q[0]))
+        rho = rho[2:-2, 2:-2]                                    # vertices of the window interior
+        div = (Ex[1:-1, 1:-1] - Ex[1:-1, 0:-2]) / hx + (Ey[1:-1, 1:-1] - Ey[0:-2, 1:-1]) / hy
+        return div - rho / eps0, (wx0, wy0), np.abs(rho).max() / eps0
+
+    r0, p0, s0 = residual()
+    m.step(3)
+    r1, p1, _ = residual()
+    dx, dy = p1[0] - p0[0], p1[1] - p0[1]
+    assert dy == 0 and dx >= 0
+    # compare on the common interior (window moved dx columns)
+    a = r0[2:-2, 2 + dx:-2]
+    bb = r1[2:-2, 2:-2 - dx] if dx else r1[2:-2, 2:-2]
+    drift = np.abs(bb - a).max() / s0
+    assert drift < tol, "Gauss-law drift %.3e (order %d fp32 %s)" % (drift, order, fp32)
