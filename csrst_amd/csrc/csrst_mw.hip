@@ -318,6 +318,29 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     constexpr int NY = ORDER == 0 ? (COMP == 0 ? 2 : 1) : (ORDER == 1 ? (COMP == 0 ? 3 : 2) : (COMP == 0 ? 4 : 3));
     const vec2<T> odt0 = pm::mk(c.odt_y, c.odt_x);   // S0: (1/(hy dt), 1/(hx dt)), ZerothOrderShapeFunction.cpp:224
     const vec2<T> qodt = pm::mk(c.qodt_x, c.qodt_y);
+    constexpr int NN = NX * NY;
+    // per-lane add of one stencil into the LDS tile (global atomics off-tile)
+    auto lane_add = [&](int I0, int J0, const T *w) {
+#pragma unroll
+        for (int t = 0; t < NN; ++t) {
+            const int gxn = I0 + (t % NX), gyn = J0 + (t / NX);
+            const int li = gxn - ax, lj = gyn - ay;
+            if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
+                atomicAdd(&acc[lj * kTW + li], w[t]);
+                tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+            } else {
+                const int i = gxn - wx0, j = gyn - wy0;
+                if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { err |= 1; continue; }
+                atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], w[t]);
+            }
+        }
+    };
+    // two register accumulation slots per lane
+    bool hasA = false, hasB = false;
+    int aI = 0, aJ = 0, bI = 0, bJ = 0;
+    T accA[NN], accB[NN];
+#pragma unroll
+    for (int t = 0; t < NN; ++t) { accA[t] = T(0); accB[t] = T(0); }
     for (int it = 0; it < kPPT; ++it) {
         const int k = base + it * kB + tid;
         const bool live = k < p.n;
@@ -373,11 +396,9 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         const int nseg = live ? xs.n - 1 : 0;
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            const bool act = s < nseg;
-            if (__ballot(act) == 0) continue;
+            if (s >= nseg) continue;
             pm::SegCurrent<T> sc;
-            sc.I0 = INT_MIN; sc.J0 = INT_MIN;
-            if (act) {
+            {
                 // slot selects instead of xs.s[s] keep the list in registers
                 const vec2<T> xi = s == 0 ? xs.s[0] : (s == 1 ? xs.s[1] : xs.s[2]);
                 const vec2<T> xf = s == 0 ? xs.s[1] : (s == 1 ? xs.s[2] : xs.s[3]);
@@ -398,42 +419,24 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
                 }
                 sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
             }
-            const unsigned long long am = __ballot(act);
-            const int lead = __ffsll((long long)am) - 1;
-            const int aI = __shfl(sc.I0, lead, 64), aJ = __shfl(sc.J0, lead, 64);
-            const bool uniform = __ballot(act && (sc.I0 != aI || sc.J0 != aJ)) == 0;
-            if (uniform) {
-                // stencil shared by the wavefront: one LDS add per node
+            // register accumulation: consecutive particles of a thread sit in
+            // the same cell of the sorted order, so their segments share anchors
+            if (hasA && sc.I0 == aI && sc.J0 == aJ) {
 #pragma unroll
-                for (int t = 0; t < NX * NY; ++t) {
-                    const T v = wave_sum(act ? sc.w[t] : T(0));
-                    if (lane == lead) {
-                        const int gxn = aI + (t % NX), gyn = aJ + (t / NX);
-                        const int li = gxn - ax, lj = gyn - ay;
-                        if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
-                            atomicAdd(&acc[lj * kTW + li], v);
-                            tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
-                        } else {
-                            const int i = gxn - wx0, j = gyn - wy0;
-                            if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
-                            else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
-                        }
-                    }
-                }
-            } else if (act) {
+                for (int t = 0; t < NN; ++t) accA[t] += sc.w[t];
+            } else if (hasB && sc.I0 == bI && sc.J0 == bJ) {
 #pragma unroll
-                for (int t = 0; t < NX * NY; ++t) {
-                    const int gxn = sc.I0 + (t % NX), gyn = sc.J0 + (t / NX);
-                    const int li = gxn - ax, lj = gyn - ay;
-                    if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
-                        atomicAdd(&acc[lj * kTW + li], sc.w[t]);
-                        tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
-                    } else {
-                        const int i = gxn - wx0, j = gyn - wy0;
-                        if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { err |= 1; continue; }
-                        atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], sc.w[t]);
-                    }
-                }
+                for (int t = 0; t < NN; ++t) accB[t] += sc.w[t];
+            } else if (!hasA) {
+                hasA = true; aI = sc.I0; aJ = sc.J0;
+#pragma unroll
+                for (int t = 0; t < NN; ++t) accA[t] = sc.w[t];
+            } else if (!hasB) {
+                hasB = true; bI = sc.I0; bJ = sc.J0;
+#pragma unroll
+                for (int t = 0; t < NN; ++t) accB[t] = sc.w[t];
+            } else {
+                lane_add(sc.I0, sc.J0, sc.w);
             }
         }
         // ---- drift back by half a step (TETMSolver.hh:210, 265) and store
@@ -447,6 +450,37 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             p.px[k] = dP.x; p.py[k] = dP.y;
         }
     }
+    // ---- flush the register slots: one DPP wavefront sum per node when every
+    // lane holding the slot shares its anchor, per-lane LDS adds otherwise
+    auto flush_slot = [&](bool has, int sI, int sJ, const T *a) {
+        const unsigned long long hm = __ballot(has);
+        if (hm == 0) return;
+        const int lead = __ffsll((long long)hm) - 1;
+        const int lI = __shfl(sI, lead, 64), lJ = __shfl(sJ, lead, 64);
+        const bool uniform = __ballot(has && (sI != lI || sJ != lJ)) == 0;
+        if (uniform) {
+#pragma unroll
+            for (int t = 0; t < NN; ++t) {
+                const T v = wave_sum(has ? a[t] : T(0));
+                if (lane == lead) {
+                    const int gxn = lI + (t % NX), gyn = lJ + (t / NX);
+                    const int li = gxn - ax, lj = gyn - ay;
+                    if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
+                        atomicAdd(&acc[lj * kTW + li], v);
+                        tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+                    } else {
+                        const int i = gxn - wx0, j = gyn - wy0;
+                        if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
+                        else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
+                    }
+                }
+            }
+        } else if (has) {
+            lane_add(sI, sJ, a);
+        }
+    };
+    flush_slot(hasA, aI, aJ, accA);
+    flush_slot(hasB, bI, bJ, accB);
     // ---- written-node box of the tile, then flush the tile with one global
     // atomic per touched node
     tminx = hu::wave_min(tminx); tminy = hu::wave_min(tminy);
@@ -629,6 +663,7 @@ __global__ void k_move(WinState *ws, int W, int H, int Wp, int Hp, int lead)
     int mi = bmaxx > W - 1 - lead ? 1 : 0;
     const int cy2 = bminy + bmaxy, mid2 = H - 1;
     int mj = cy2 > mid2 + 2 ? 1 : (cy2 < mid2 - 2 ? -1 : 0);
+    if (lead < 0) { mi = 0; mj = 0; }   // footprint update only
     ws->mi = mi; ws->mj = mj;
     ws->wx0 += mi; ws->wy0 += mj;
     ws->rx = (ws->rx + mi) % Wp;
@@ -1158,9 +1193,9 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
         if (f32) pf = m->parts<float>(m->partBuf); else pd = m->parts<double>(m->partBuf);
         mw::k_bbox_init<<<(n + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(pf, pd, f32, m->ws.p);
         // initial footprint = particle box +- 4 (k_move convention)
-        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, 1 << 30);
+        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, -1);
         HU_LAUNCH_CHECK();
-        // k_move with a huge lead makes no move; reset the move counters
+        // k_move with a negative lead makes no move; reset the counters
         mw::WinState h;
         HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
         HU_CHECK(hipStreamSynchronize(m->s));

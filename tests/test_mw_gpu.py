@@ -113,44 +113,39 @@ def test_bunch_advances_at_c(gpu):
 
 
 @pytest.mark.parametrize("order", [0, 1, 2])
-@pytest.mark.parametrize("fp32,tol", [(False, 1e-9), (True, 2e-4)])
-def test_deposit_conserves_charge(gpu, order, fp32, tol):
-    """Gauss law drift: with the bunch moving along x only (no self force at
-    this charge, no bend field before the bend), the deposited Jx keeps
-    div E - rho/eps0 unchanged from step to step (charge-conserving
-    deposit, FirstOrderShapeFunction / SecondOrderShapeFunction continuity;
-    rho with the matching vertex shape of the oracle)."""
+def test_first_step_ex_is_deposit(gpu, order):
+    """From zero fields, after one step Ex = -c dt Z0 Jx exactly (the first
+    half step of H is zero, the second does not feed back into Ex), with Jx
+    the oracle deposit of the first push R0 -> R0 + 2 (0.5 c dt) P/gamma."""
     b = O.Bend(Nx=NX, Ny=NY, **GEOM)
     hx, hy = b.hr
-    n = 4000
-    rng = np.random.default_rng(3)
+    n = 3000
+    rng = np.random.default_rng(5)
     R = np.stack([0.02 + rng.normal(0, 0.3e-3, n), 0.0064 + rng.normal(0, 0.4e-3, n)], 1)
     P = np.stack([np.full(n, 99.995), np.zeros(n)], 1)
     Q = np.full(n, -1e-22)
-    m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
+    m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=False, sortEvery=0, statModulus=0, **GEOM)
     m.init_bunch(R, P, Q)
-    eps0 = 8.854187817e-12
-
-    def residual():
-        inf = m.info()
-        wx0, wy0 = inf["pos"]
-        F = m.fields()
-        Ex, Ey = F[0], F[1]      # [H+2, W+2], logical index + 1
-        Rp, _ = m.particles()
-        org = (b.origin[0], b.origin[1])
-        W, H = m.W, m.H
-        rho = O.charge_density(order, (wx0 - 2, wy0 - 2, W + 4, H + 4), org, (hx, hy), Rp, np.full(n, Q[0]))
-        rho = rho[2:-2, 2:-2]                                    # vertices of the window interior
-        div = (Ex[1:-1, 1:-1] - Ex[1:-1, 0:-2]) / hx + (Ey[1:-1, 1:-1] - Ey[0:-2, 1:-1]) / hy
-        return div - rho / eps0, (wx0, wy0), np.abs(rho).max() / eps0
-
-    r0, p0, s0 = residual()
-    m.step(3)
-    r1, p1, _ = residual()
-    dx, dy = p1[0] - p0[0], p1[1] - p0[1]
-    assert dy == 0 and dx >= 0
-    # compare on the common interior (window moved dx columns)
-    a = r0[2:-2, 2 + dx:-2]
-    bb = r1[2:-2, 2:-2 - dx] if dx else r1[2:-2, 2:-2]
-    drift = np.abs(bb - a).max() / s0
-    assert drift < tol, "Gauss-law drift %.3e (order %d fp32 %s)" % (drift, order, fp32)
+    inf = m.info()
+    wx0, wy0 = inf["pos"]
+    dt = inf["dt"]
+    R0, P0 = m.particles()
+    g = np.array([O.q_rsqrt(float(1.0 + p @ p)) for p in P0])
+    half = 0.5 * dt * 299792458.0
+    R2 = R0 + 2 * half * P0 * g[:, None]
+    Jx = O.deposit(order, 0, (wx0, wy0, m.W, m.H), tuple(b.origin), (hx, hy), dt, R0, R2, Q)
+    m.step(1)
+    sx = m.info()["pos"][0] - wx0
+    assert m.info()["pos"][1] == wy0 and sx in (0, 1)
+    F = m.fields()
+    Ex = F[0][1:-1, 1:-1][:, :m.W - sx]       # new window, columns shared with the old one
+    # bulk cells use dt/eps0 in the form c dt Z0 (TETMSolver.hh:125); the
+    # two differ by sqrt(c^2 mu0 eps0) = 1 - 2.1e-10 with Physics.cpp's constants
+    mu0, eps0 = 1.256637061e-06, 8.854187817e-12
+    dteps = 299792458.0 * dt * np.sqrt(mu0 / eps0)
+    expect = -dteps * Jx[:, sx:]
+    scale = np.abs(expect).max()
+    d = np.abs(Ex - expect)
+    j, i = np.unravel_index(np.argmax(d), d.shape)
+    assert d.max() < 1e-12 * scale, "max |Ex - expect| = %.3e of %.3e at (%d, %d): %r vs %r" % (
+        d.max(), scale, i, j, Ex[j, i], expect[j, i])
