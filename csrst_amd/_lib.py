@@ -25,6 +25,7 @@ SYMBOLS = [
     "csrst_mw_upload_fields",
     "csrst_mw_download_fields", "csrst_mw_download_particles", "csrst_mw_info", "csrst_mw_wake",
     "csrst_mw_timing",
+    "csrst_poisson_solve", "csrst_poisson_compute_field",
 ]
 
 

@@ -940,13 +940,12 @@ struct csrst_mw {
 
 namespace {
 
-thread_local std::string g_mw_err;
 
 int mw_fail(csrst_mw *m, const std::exception &e)
 {
     const Error *he = dynamic_cast<const Error *>(&e);
     if (m) m->err = e.what();
-    g_mw_err = e.what();
+    hu::global_error() = e.what();
     return he ? he->code : CSRST_EINVAL;
 }
 

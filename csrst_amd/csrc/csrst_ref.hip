@@ -26,6 +26,7 @@
 #include "bend_geom.h"
 #include "hip_util.h"
 #include "pic_math.h"
+#include "poisson_mg.h"
 
 using hu::DBuf;
 using hu::Error;
@@ -556,6 +557,125 @@ struct Rect {
 }  // namespace
 
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Window copy of main.cpp:335-337 (Communicator.hh:211-272): global E and H
+// into the window over the overlap of the two index ranges.
+__global__ void k_copy_to_window(FieldDesc g, FieldDesc w, int s0, int s1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // window column
+    const int j = blockIdx.y;                               // window row
+    if (i >= w.n || j >= w.m) return;
+    const int gi = i + s0, gj = j + s1;
+    if (gi < 0 || gj < 0 || gi >= g.n || gj >= g.m) return;
+    w.c0[fidx(w, i, j)] = g.c0[fidx(g, gi, gj)];
+    w.c1[fidx(w, i, j)] = g.c1[fidx(g, gi, gj)];
+}
+
+// ---------------------------------------------------------------------------
+// Poisson initialisation (PoissonSolver.cpp:85-201).
+
+// PartBunch::scatterQ (ShapeFunction getChargeDensity): vertex charge of each
+// particle onto the global grid, FP64 atomics (the Poisson start is a solve
+// to a tolerance, so the summation order does not matter).
+template <int ORDER>
+__global__ void k_scatter_q(Parts p, double o0, double o1, double h0, double h1, double *rho, int nx, int ny)
+{
+    constexpr int N = ORDER + 2;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.n) return;
+    const double q = p.q[k];
+    double wx[N], wy[N];
+    int I0, J0;
+    pm::charge_weights<ORDER>(pm::mk((p.rx[k] - o0) / h0, (p.ry[k] - o1) / h1), q, wx, wy, I0, J0);
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            const int I = I0 + a, J = J0 + b;
+            if (I < 0 || J < 0 || I >= nx || J >= ny) continue;
+            unsafeAtomicAdd(&rho[(size_t)J * nx + I], pm::charge_w<ORDER>(q, wx[a], wy[b]));
+        }
+}
+
+// fillRHS (PoissonSolver.cpp:441-460): b = rho / (hx hy) / (eps0 gamma).
+__global__ void k_rhs(const double *rho, double *b, size_t n, double hh, double cc)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = (0.0 + rho[k] / hh) * cc;
+}
+
+__global__ void k_move_x(Parts p, double dx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < p.n) p.rx[k] = p.rx[k] + dx;
+}
+
+__global__ void k_save_r(Parts p, double *sx, double *sy, int dir)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.n) return;
+    if (dir == 0) { sx[k] = p.rx[k]; sy[k] = p.ry[k]; }
+    else { p.rx[k] = sx[k]; p.ry[k] = sy[k]; }
+}
+
+// shiftLHS (PoissonSolver.cpp:279-319): next initial guess from the last
+// potential, phi_i + tau (phi_{i+1} - phi_i) for tau > 0, with phi_{i-1}
+// for tau < 0; values past the grid read as 0.
+__global__ void k_shift_guess(const double *phi, double *out, int nx, int ny, double tau)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)nx * ny) return;
+    const int i = (int)(k % nx);
+    const double v = phi[k];
+    if (tau > 0) {
+        const double t = tau > 1 ? 1.0 : tau;
+        const double nb = i + 1 < nx ? phi[k + 1] : 0.0;
+        out[k] = v + t * nb - t * v;
+    } else {
+        double t = -tau;
+        if (t > 1) t = 1;
+        const double nb = i > 0 ? phi[k - 1] : 0.0;
+        out[k] = v + t * nb - t * v;
+    }
+}
+
+// fillFields (PoissonSolver.cpp:462-507): forward differences of the potential.
+// comp 0: Ex = phi_i/hr0 - phi_{i+1}/hr0; 1: Ey = g phi_j/hy - g phi_{j+1}/hy;
+// 2: both Hz halves = bg phi_j/(2 c mu0 hy) - bg phi_{j+1}/(2 c mu0 hy).
+__global__ void k_fill_fields(const double *phi, int comp, FieldDesc f, double scale, double den)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = f.n, ny = f.m;
+    if (k >= (size_t)nx * ny) return;
+    const int i = (int)(k % nx), j = (int)(k / nx);
+    if (comp == 0) {
+        const double nb = i + 1 < nx ? phi[k + 1] : 0.0;
+        f.c0[fidx(f, i, j)] = phi[k] / den - nb / den;
+    } else {
+        const double nb = j + 1 < ny ? phi[k + nx] : 0.0;
+        const double v = scale * phi[k] / den - scale * nb / den;
+        if (comp == 1) {
+            f.c1[fidx(f, i, j)] = v;
+        } else {
+            f.c0[fidx(f, i, j)] = v;
+            f.c1[fidx(f, i, j)] = v;
+        }
+    }
+}
+
+// SCBoundaryCondition::resetOutsideDomain (SCBoundaryCondition.hh:164-196):
+// per cell bit 0 clears Ex, bit 1 clears Ey.
+__global__ void k_reset_outside(FieldDesc E, const uint8_t *code)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)E.n * E.m) return;
+    const uint8_t c = code[k];
+    if (!c) return;
+    const int i = (int)(k % E.n), j = (int)(k / E.n);
+    if (c & 1) E.c0[fidx(E, i, j)] = 0.0;
+    if (c & 2) E.c1[fidx(E, i, j)] = 0.0;
+}
+
 struct csrst_ctx {
     std::string err;
     csrst_params p{};
@@ -593,6 +713,8 @@ struct csrst_ctx {
     // diagnostics
     DBuf<DD> partial, wacc;
     DBuf<double> mom;
+    // Poisson start: node classes and resetOutsideDomain codes
+    DBuf<uint8_t> pcls, preset;
     int steps = 0;
     bool ready = false;
 
@@ -610,14 +732,12 @@ struct csrst_ctx {
 
 namespace {
 
-thread_local std::string g_err;
-
 int fail(csrst_ctx *c, const std::exception &e)
 {
     const Error *he = dynamic_cast<const Error *>(&e);
     const std::string m = e.what();
     if (c) c->err = m;
-    g_err = m;
+    hu::global_error() = m;
     return he ? he->code : CSRST_EINVAL;
 }
 
@@ -973,6 +1093,125 @@ void move_mesh(csrst_ctx *c, int *mvI, int *mvJ)
     if (mvJ) *mvJ = sj;
 }
 
+void copy_to_window(csrst_ctx *c)
+{
+    for (int f = 0; f < 2; ++f) {
+        const FieldDesc &gd = c->fs.f[f == 0 ? F_GE : F_GH];
+        const FieldDesc &wd = c->fs.f[f == 0 ? F_ME : F_MH];
+        dim3 grid((unsigned)((c->W + kBlock - 1) / kBlock), (unsigned)c->H);
+        k_copy_to_window<<<grid, kBlock, 0, c->s>>>(gd, wd, c->ddim[0][0], c->ddim[1][0]);
+    }
+    HU_LAUNCH_CHECK();
+}
+
+// Node classes of PoissonSolver::cutoffStencil (PoissonSolver.cpp:678-766)
+// and the resetOutsideDomain codes, from the host geometry (once).
+void poisson_geometry(csrst_ctx *c)
+{
+    if (!c->pcls.p) {
+        const int Nx = c->p.Nx, Ny = c->p.Ny;
+        const size_t nn = (size_t)Nx * Ny;
+        std::vector<int> bmap(nn, -1);
+        const int nb = (int)(c->hbij.size() / 2);
+        for (int k = 0; k < nb; ++k) bmap[(size_t)c->hbij[2 * k + 1] * Nx + c->hbij[2 * k]] = k;
+        const long maxX = (long)(size_t)(c->p.lenBefore / c->hr[0]);    // Bend.hh:288-291
+        std::vector<uint8_t> cls(nn), code(nn);
+        for (int j = 0; j < Ny; ++j)
+            for (int i = 0; i < Nx; ++i) {
+                const size_t id = (size_t)j * Nx + i;
+                const int bc = bmap[id];
+                const bool flag = c->hmask[id] || (bc > -1 && c->hbdata[5 * bc + 2] > 0.5);
+                cls[id] = flag ? (i < maxX ? pmg::kUnknown : pmg::kKnown) : pmg::kFixed;
+                uint8_t r = 0;
+                if (!c->hmask[id]) {
+                    if (bc == -1) {
+                        r = 3;
+                    } else {
+                        if (c->hbdata[5 * bc] < pm::kSpaceEps) r |= 1;
+                        if (c->hbdata[5 * bc + 1] < pm::kSpaceEps) r |= 2;
+                        if (c->hbdata[5 * bc + 2] < pm::kMinArea) r = 3;
+                    }
+                }
+                code[id] = r;
+            }
+        c->pcls.alloc(nn);
+        c->preset.alloc(nn);
+        HU_CHECK(hipMemcpy(c->pcls.p, cls.data(), nn, hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->preset.p, code.data(), nn, hipMemcpyHostToDevice));
+    }
+}
+
+// PoissonSolver::computeField (PoissonSolver.cpp:85-201), resetOutsideDomain
+// and the window copy and exchanges that follow it in main.cpp:229-341.
+void poisson_compute_field(csrst_ctx *c, double tol, int maxit, int *cycles, double *resid)
+{
+    const int Nx = c->p.Nx, Ny = c->p.Ny;
+    const size_t nn = (size_t)Nx * Ny;
+    poisson_geometry(c);
+    // gamma from the mean momentum (PoissonSolver.cpp:64-66)
+    moments(c);
+    double mom[5];
+    HU_CHECK(hipMemcpyAsync(mom, c->mom.p, sizeof(mom), hipMemcpyDeviceToHost, c->s));
+    HU_CHECK(hipStreamSynchronize(c->s));
+    const double gamma = sqrt(mom[1] * mom[1] + mom[3] * mom[3] + 1);
+    const double hr0 = c->hr[0] * gamma, hr1 = c->hr[1];
+    DBuf<double> rho, b, phi, guess, sx, sy;
+    rho.alloc(nn); b.alloc(nn); phi.alloc(nn); guess.alloc(nn);
+    sx.alloc(c->n); sy.alloc(c->n);
+    pmg::Solver mg;
+    mg.setup(Nx, Ny, hr0, hr1, c->pcls.p, c->s);
+    const double cc = 1 / (pm::kEps0 * gamma);
+    auto rhs = [&]() {
+        rho.zero(c->s);
+        const double o0 = c->bend.origin[0], o1 = c->bend.origin[1];
+        if (c->p.order == 0)
+            k_scatter_q<0><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, c->hr[0], c->hr[1], rho.p, Nx, Ny);
+        else if (c->p.order == 1)
+            k_scatter_q<1><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, c->hr[0], c->hr[1], rho.p, Nx, Ny);
+        else
+            k_scatter_q<2><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, c->hr[0], c->hr[1], rho.p, Nx, Ny);
+        k_rhs<<<blocks(nn), kBlock, 0, c->s>>>(rho.p, b.p, nn, c->hr[0] * c->hr[1], cc);
+        HU_LAUNCH_CHECK();
+    };
+    auto solve = [&](int k) {
+        const pmg::Stats st = mg.solve(b.p, phi.p, tol, maxit);
+        if (cycles) cycles[k] = st.cycles;
+        if (resid) resid[k] = st.bnorm > 0 ? st.resid / st.bnorm : st.resid;
+    };
+    k_save_r<<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, sx.p, sy.p, 0);          // saveR
+    // solve 1: Ey, charge at t + dt/2 (PoissonSolver.cpp:154-166)
+    drift(c, c->dt / 2);
+    rhs();
+    drift(c, -c->dt / 2);
+    phi.zero(c->s);
+    solve(0);
+    FieldDesc E = c->fs.f[F_GE], H = c->fs.f[F_GH];
+    k_fill_fields<<<blocks(nn), kBlock, 0, c->s>>>(phi.p, 1, E, gamma, hr1);
+    // solve 2: Hz, bunch half a cell back (PoissonSolver.cpp:167-178)
+    k_move_x<<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, -0.5 * hr0 / gamma);
+    rhs();
+    k_shift_guess<<<blocks(nn), kBlock, 0, c->s>>>(phi.p, guess.p, Nx, Ny, 1.0);
+    std::swap(phi.p, guess.p);
+    solve(1);
+    const double betaGamma = sqrt(gamma * gamma - 1);
+    k_fill_fields<<<blocks(nn), kBlock, 0, c->s>>>(phi.p, 2, H, betaGamma, 2 * pm::kC * pm::kMu0 * hr1);
+    // solve 3: Ex (PoissonSolver.cpp:179-195)
+    k_save_r<<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, sx.p, sy.p, 1);          // restoreR
+    rhs();
+    k_shift_guess<<<blocks(nn), kBlock, 0, c->s>>>(phi.p, guess.p, Nx, Ny, -0.5);
+    std::swap(phi.p, guess.p);
+    solve(2);
+    k_fill_fields<<<blocks(nn), kBlock, 0, c->s>>>(phi.p, 0, E, 1.0, hr0);
+    k_reset_outside<<<blocks(nn), kBlock, 0, c->s>>>(E, c->preset.p);
+    HU_LAUNCH_CHECK();
+    c->gJ.zero(c->s);                                                               // main.cpp:374
+    copy_to_window(c);
+    exchange(c, 0);
+    exchange(c, 1);
+    exchange(c, 2);
+    HU_CHECK(hipStreamSynchronize(c->s));
+}
+
 void full_step(csrst_ctx *c)
 {
     first_step(c);
@@ -994,7 +1233,7 @@ const char *csrst_last_error(const void *ctx)
 {
     // both handle types start with the error string member
     if (ctx) return static_cast<const csrst_ctx *>(ctx)->err.c_str();
-    return g_err.c_str();
+    return hu::global_error().c_str();
 }
 
 int csrst_device_count(int *n)
@@ -1178,27 +1417,7 @@ int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const doubl
         HU_CHECK(hipcub::DeviceRadixSort::SortPairs((void *)nullptr, c->cubBytes, c->keys.p, c->keys2.p, c->vals.p,
                                                     c->vals2.p, (int)cnt, 0, 32, c->s));
         c->cubTmp.alloc(c->cubBytes + 16);
-        // copyFields global -> window (main.cpp:335-337, Communicator.hh:211-272)
-        {
-            std::vector<double> g((size_t)2 * (c->p.Nx + 2) * (c->p.Ny + 2)), w((size_t)2 * (c->W + 2) * (c->H + 2), 0.0);
-            for (int f = 0; f < 2; ++f) {
-                DBuf<double> &gb = f == 0 ? c->gE : c->gH;
-                DBuf<double> &wb = f == 0 ? c->mEb : c->mHb;
-                HU_CHECK(hipMemcpy(g.data(), gb.p, gb.bytes(), hipMemcpyDeviceToHost));
-                HU_CHECK(hipMemcpy(w.data(), wb.p, wb.bytes(), hipMemcpyDeviceToHost));
-                const FieldDesc &gd = c->fs.f[f == 0 ? F_GE : F_GH];
-                const FieldDesc &wd = c->fs.f[f == 0 ? F_ME : F_MH];
-                const size_t gpl = gb.n / 2, wpl = wb.n / 2;
-                const int s0 = c->ddim[0][0], s1 = c->ddim[1][0];
-                const int lo0 = std::max(0, s0), hi0 = std::min(c->p.Nx - 1, c->W - 1 + s0);
-                const int lo1 = std::max(0, s1), hi1 = std::min(c->p.Ny - 1, c->H - 1 + s1);
-                for (int j = lo1; j <= hi1; ++j)
-                    for (int i = lo0; i <= hi0; ++i)
-                        for (int k = 0; k < 2; ++k)
-                            w[k * wpl + fidx(wd, i - s0, j - s1)] = g[k * gpl + fidx(gd, i, j)];
-                HU_CHECK(hipMemcpy(wb.p, w.data(), wb.bytes(), hipMemcpyHostToDevice));
-            }
-        }
+        copy_to_window(c);
         exchange(c, 0);
         exchange(c, 1);
         exchange(c, 2);
@@ -1209,6 +1428,19 @@ int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const doubl
         c->steps = 0;
         c->ready = true;
         HU_CHECK(hipStreamSynchronize(c->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_poisson_compute_field(csrst_ctx *c, double tol, int maxit, int *cycles, double *resid)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "csrst_poisson_compute_field before csrst_init_bunch");
+        if (c->steps != 0) throw Error(CSRST_ESTATE, "the Poisson start must precede the first step");
+        if (maxit < 0 || !(tol >= 0)) throw Error(CSRST_EINVAL, "bad tolerance or iteration limit");
+        poisson_compute_field(c, tol, maxit, cycles, resid);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);

@@ -27,6 +27,14 @@ struct Error : std::runtime_error {
 
 #define HU_LAUNCH_CHECK() HU_CHECK(hipGetLastError())
 
+// Last error of a call that had no handle to store it in (one per thread,
+// shared by every translation unit of the library).
+inline std::string &global_error()
+{
+    static thread_local std::string e;
+    return e;
+}
+
 inline std::string fmt(const char *f, ...)
 {
     char buf[1024];

@@ -455,50 +455,52 @@ PM_HD void segment_current(SegCurrent<T> &o, int order, int comp, vec2<T> xi, ve
 }
 
 // ---------------------------------------------------------------------------
-// Vertex charge weights ({Zeroth,First,Second}OrderShapeFunction.cpp
-// getChargeDensityImpl); `ne` is the position in mesh units.
-template <class T>
-struct ChargeStencil {
-    int I0, J0, n;
-    T w[16];
-};
-
-template <class T>
-PM_HD void charge_stencil(ChargeStencil<T> &s, int order, vec2<T> ne, T q)
+// Vertex charge weights of getChargeDensityImpl: ORDER 0 CIC
+// (ZerothOrderShapeFunction.cpp:344-357), 1 TSC about the nearest vertex
+// (FirstOrderShapeFunction.cpp:389-413), 2 cubic B-spline
+// (SecondOrderShapeFunction.cpp:393-427).  ne = (R - origin) / h.  The node
+// (I0 + a, J0 + b) receives charge_w<ORDER>(q, wx[a], wy[b]); each order
+// keeps its reference's association of the product.
+template <int ORDER, class T>
+PM_HD void charge_weights(vec2<T> ne, T q, T (&wx)[ORDER + 2], T (&wy)[ORDER + 2], int &I0, int &J0)
 {
-    if (order == 0) {
+    if constexpr (ORDER == 0) {
         const int I = (int)floor_(ne.x), J = (int)floor_(ne.y);
         const T tx = ne.x - (T)I, ty = ne.y - (T)J;
-        s.I0 = I; s.J0 = J; s.n = 2;
-        s.w[0] = (T(1.0) - tx) * (T(1.0) - ty) * q;
-        s.w[1] = tx * (T(1.0) - ty) * q;
-        s.w[2] = (T(1.0) - tx) * ty * q;
-        s.w[3] = tx * ty * q;
-    } else if (order == 1) {
+        I0 = I; J0 = J;
+        wx[0] = T(1.0) - tx; wx[1] = tx;
+        wy[0] = T(1.0) - ty; wy[1] = ty;
+    } else if constexpr (ORDER == 1) {
         const int I = (int)floor_(ne.x + T(0.5)), J = (int)floor_(ne.y + T(0.5));
         const T a = ne.x - (T)I, b = ne.y - (T)J;
         const T sxm = q * (T(0.5) - a) * (T(0.5) - a) * T(0.5);
         const T sxp = q * (T(0.5) + a) * (T(0.5) + a) * T(0.5);
-        const T sx = q - sxm - sxp;
+        wx[0] = sxm; wx[1] = q - sxm - sxp; wx[2] = sxp;
         const T sym = (T(0.5) - b) * (T(0.5) - b) * T(0.5);
         const T syp = (T(0.5) + b) * (T(0.5) + b) * T(0.5);
-        const T sy = T(1.0) - sym - syp;
-        const T wx[3] = {sxm, sx, sxp}, wy[3] = {sym, sy, syp};
-        s.I0 = I - 1; s.J0 = J - 1; s.n = 3;
-        for (int bj = 0; bj < 3; ++bj)
-            for (int ai = 0; ai < 3; ++ai) s.w[bj * 3 + ai] = wx[ai] * wy[bj];
+        wy[0] = sym; wy[1] = T(1.0) - sym - syp; wy[2] = syp;
+        I0 = I - 1; J0 = J - 1;
     } else {
         const T o6 = T(1.0 / 6.0);
         const int I = (int)floor_(ne.x), J = (int)floor_(ne.y);
         const T a = ne.x - (T)I, b = ne.y - (T)J;
-        const T wx[4] = {o6 * ((T(1.0) - a) * (T(1.0) - a) * (T(1.0) - a)), o6 * (T(4.0) - T(3.0) * a * a * (T(2.0) - a)),
-                         o6 * (T(4.0) - T(3.0) * (T(1.0) - a) * (T(1.0) - a) * (T(1.0) + a)), o6 * (a * a * a)};
-        const T wy[4] = {o6 * ((T(1.0) - b) * (T(1.0) - b) * (T(1.0) - b)), o6 * (T(4.0) - T(3.0) * b * b * (T(2.0) - b)),
-                         o6 * (T(4.0) - T(3.0) * (T(1.0) - b) * (T(1.0) - b) * (T(1.0) + b)), o6 * (b * b * b)};
-        s.I0 = I - 1; s.J0 = J - 1; s.n = 4;
-        for (int bj = 0; bj < 4; ++bj)
-            for (int ai = 0; ai < 4; ++ai) s.w[bj * 4 + ai] = q * wx[ai] * wy[bj];
+        wx[0] = o6 * ((T(1.0) - a) * (T(1.0) - a) * (T(1.0) - a));
+        wx[1] = o6 * (T(4.0) - T(3.0) * a * a * (T(2.0) - a));
+        wx[2] = o6 * (T(4.0) - T(3.0) * (T(1.0) - a) * (T(1.0) - a) * (T(1.0) + a));
+        wx[3] = o6 * (a * a * a);
+        wy[0] = o6 * ((T(1.0) - b) * (T(1.0) - b) * (T(1.0) - b));
+        wy[1] = o6 * (T(4.0) - T(3.0) * b * b * (T(2.0) - b));
+        wy[2] = o6 * (T(4.0) - T(3.0) * (T(1.0) - b) * (T(1.0) - b) * (T(1.0) + b));
+        wy[3] = o6 * (b * b * b);
+        I0 = I - 1; J0 = J - 1;
     }
+}
+template <int ORDER, class T>
+PM_HD T charge_w(T q, T wx, T wy)
+{
+    if constexpr (ORDER == 0) return wx * wy * q;
+    else if constexpr (ORDER == 1) return wx * wy;
+    else return q * wx * wy;
 }
 
 // ---------------------------------------------------------------------------

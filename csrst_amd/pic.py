@@ -93,6 +93,16 @@ class TwoGridStep:
         self.n = len(R)
         check(_lib.lib().csrst_init_bunch(self.h, dptr(R), dptr(P), dptr(Q), C.c_int(self.n)), self.h)
 
+    def computeField(self, tol=1e-12, maxit=500):
+        """PoissonSolver::computeField (PoissonSolver.cpp:85-201) on the GPU:
+        the three multigrid solves into the global E and H, resetOutsideDomain
+        and the window copy (main.cpp:229-341).  Call after init_bunch, before
+        the first step.  Returns the V-cycles and relative residual per solve."""
+        cyc = (C.c_int * 3)()
+        res = (C.c_double * 3)()
+        check(_lib.lib().csrst_poisson_compute_field(self.h, C.c_double(tol), C.c_int(maxit), cyc, res), self.h)
+        return dict(cycles=list(cyc), resid=list(res))
+
     # -- stepping ------------------------------------------------------------
     def step(self, n=1):
         check(_lib.lib().csrst_step(self.h, C.c_int(n)), self.h)

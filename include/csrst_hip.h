@@ -132,6 +132,29 @@ int csrst_mw_wake(csrst_mw *m, double *out, int cap, int *count);
  * [0] push+deposit, [1] field sweeps, [2] window move, [3] sort, [4] wake (ms). */
 int csrst_mw_timing(csrst_mw *m, double *ms, int cap);
 
+/* ---- Poisson initialisation ------------------------------------------------
+ * Replaces PoissonSolver::computeField (PoissonSolver.cpp:85-201), whose
+ * Belos block-CG + ML AMG solves the cutoffStencil system
+ * (PoissonSolver.cpp:678-766); here FP64 geometric multigrid on the GPU.
+ *
+ * csrst_poisson_solve: one solve on host arrays over an nx x ny node grid
+ * (row-major, x fastest).  cls per node: 0 = identity row (phi = b, not
+ * coupled), 1 = unknown (5-point row), 2 = identity row coupled to unknown
+ * neighbours (its b acts as a boundary value).  hx is the x spacing already
+ * scaled by gamma (PoissonSolver.cpp:71).  phi: in = initial guess, out =
+ * solution.  Stops when ||b - A phi|| <= tol ||b|| over unknown rows or after
+ * maxit V-cycles; reports the cycles used and the final relative residual. */
+/* csrst_poisson_compute_field: after csrst_init_bunch and before the first
+ * step, the three solves of PoissonSolver::computeField on the global grid
+ * (Ey from the charge at t + dt/2, Hz with the bunch half a cell back, Ex),
+ * fillFields, resetOutsideDomain, then the window copy and exchanges as in
+ * main.cpp:229-341.  cycles[3], resid[3]: V-cycles and relative residual of
+ * each solve (either may be NULL).  The reference runs solve 1 to 1e-12 and
+ * solves 2 and 3 to 1e-6 from a random start; here every solve runs to tol. */
+int csrst_poisson_compute_field(csrst_ctx *c, double tol, int maxit, int *cycles, double *resid);
+int csrst_poisson_solve(int nx, int ny, double hx, double hy, const uint8_t *cls, const double *b,
+                        double *phi, double tol, int maxit, int *cycles, double *resid);
+
 #ifdef __cplusplus
 }
 #endif

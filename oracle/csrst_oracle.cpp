@@ -1011,6 +1011,112 @@ void op_invert(OpKind k, VField &f, int i, int j, double v)
     else f.at(1, i, j) = v;
 }
 
+/* ---------------------------------------------------------------------- */
+/* Poisson initialisation, PoissonSolver.cpp:85-201, 204-233, 441-507,      */
+/* 678-766.  The reference solves A phi = rho/(eps0 gamma) with Belos CG +  */
+/* ML AMG from a random start; the oracle solves the same linear system     */
+/* directly (banded Cholesky), so it carries only rounding error.           */
+
+/* Node classes of PoissonSolver::cutoffStencil (PoissonSolver.cpp:678-766):
+ * flag = insideMask || (boundary cell with area > 0.5); a row is a stencil
+ * row when idx < maxX and flag, otherwise an identity row (phi = rhs). */
+void poisson_classes(const Bend &b, std::vector<uint8_t> &flag, std::vector<uint8_t> &active)
+{
+    const long maxX = (long)(size_t)(b.lenBefore / b.hr[0]);   /* Bend.hh:288-291 */
+    flag.assign((size_t)b.Nx * b.Ny, 0);
+    active.assign((size_t)b.Nx * b.Ny, 0);
+    for (int j = 0; j < b.Ny; ++j)
+        for (int i = 0; i < b.Nx; ++i) {
+            const size_t id = (size_t)j * b.Nx + i;
+            const int bc = b.bmap[id];
+            const bool f = b.mask[id] || (bc > -1 && b.bcells[bc].area > 0.5);
+            flag[id] = f ? 1 : 0;
+            active[id] = (f && i < maxX) ? 1 : 0;
+        }
+}
+
+/* Solve the cutoffStencil system on an nx x ny node grid (row-major, x
+ * fastest).  Stencil rows: -2(EW+NS) on the diagonal, EW = -1/hx^2 to x
+ * neighbours and NS = -1/hy^2 to y neighbours whose flag is set (neighbours
+ * off the grid count as unflagged).  Identity rows: phi = b.  A flagged
+ * neighbour that is an identity row is a known value and moves to the
+ * right-hand side.  The stencil rows form an SPD system, factored by a
+ * banded Cholesky in column order (y fastest, bandwidth = box height). */
+int poisson_direct(int nx, int ny, double hx, double hy, const uint8_t *flag, const uint8_t *active,
+                   const double *b, double *phi)
+{
+    const double EW = -1 / (hx * hx);
+    const double NS = -1 / (hy * hy);
+    int i0 = nx, i1 = -1, j0 = ny, j1 = -1;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            if (active[(size_t)j * nx + i]) {
+                i0 = std::min(i0, i); i1 = std::max(i1, i);
+                j0 = std::min(j0, j); j1 = std::max(j1, j);
+            }
+    for (size_t k = 0; k < (size_t)nx * ny; ++k) phi[k] = b[k];
+    if (i1 < 0) return 0;
+    const int bw = j1 - j0 + 1, bx = i1 - i0 + 1;
+    const long n = (long)bw * bx;
+    const int p = bw;                                   /* bandwidth */
+    std::vector<double> L((size_t)n * (p + 1), 0.0);    /* L[k][d] = L(k, k-d) */
+    std::vector<double> rhs(n, 0.0);
+    auto kid = [&](int i, int j) { return (long)(i - i0) * bw + (j - j0); };
+    auto isflag = [&](int i, int j) { return i >= 0 && j >= 0 && i < nx && j < ny && flag[(size_t)j * nx + i]; };
+    auto isact = [&](int i, int j) { return i >= 0 && j >= 0 && i < nx && j < ny && active[(size_t)j * nx + i]; };
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j) {
+            const long k = kid(i, j);
+            double *Lk = &L[(size_t)k * (p + 1)];
+            if (!active[(size_t)j * nx + i]) { Lk[0] = 1.0; rhs[k] = 0.0; continue; }
+            Lk[0] = -2 * (EW + NS);
+            rhs[k] = b[(size_t)j * nx + i];
+            const int ni[4] = {i, i - 1, i + 1, i}, nj[4] = {j - 1, j, j, j + 1};
+            const double w[4] = {NS, EW, EW, NS};
+            for (int q = 0; q < 4; ++q) {
+                if (!isflag(ni[q], nj[q])) continue;
+                if (isact(ni[q], nj[q])) {
+                    const long m = kid(ni[q], nj[q]);
+                    if (m < k) Lk[k - m] = w[q];               /* lower triangle only */
+                } else {
+                    rhs[k] -= w[q] * b[(size_t)nj[q] * nx + ni[q]];
+                }
+            }
+        }
+    /* in-place banded Cholesky */
+    for (long k = 0; k < n; ++k) {
+        double *Lk = &L[(size_t)k * (p + 1)];
+        const long m0 = std::max(0L, k - p);
+        for (long m = m0; m <= k; ++m) {
+            const double *Lm = &L[(size_t)m * (p + 1)];
+            double s = Lk[k - m];
+            const long t0 = std::max(m0, std::max(0L, m - p));
+            for (long t = t0; t < m; ++t) s -= Lk[k - t] * Lm[m - t];
+            if (m == k) {
+                if (!(s > 0)) return -1;
+                Lk[0] = std::sqrt(s);
+            } else {
+                Lk[k - m] = s / Lm[0];
+            }
+        }
+    }
+    for (long k = 0; k < n; ++k) {                      /* L y = rhs */
+        const double *Lk = &L[(size_t)k * (p + 1)];
+        double s = rhs[k];
+        for (long t = std::max(0L, k - p); t < k; ++t) s -= Lk[k - t] * rhs[t];
+        rhs[k] = s / Lk[0];
+    }
+    for (long k = n - 1; k >= 0; --k) {                 /* L^T x = y */
+        double s = rhs[k];
+        for (long m = k + 1; m <= std::min(n - 1, k + p); ++m) s -= L[(size_t)m * (p + 1) + (m - k)] * rhs[m];
+        rhs[k] = s / L[(size_t)k * (p + 1)];
+    }
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j)
+            if (active[(size_t)j * nx + i]) phi[(size_t)j * nx + i] = rhs[kid(i, j)];
+    return 0;
+}
+
 struct Sim {
     orc_sim_params p;
     Bend *geo = nullptr;
@@ -1637,6 +1743,90 @@ struct Sim {
             }
     }
 
+    /* PartBunch::scatterQ onto the global vertex grid (ShapeFunction
+     * getChargeDensity, charge_one above); out-of-grid deposits are dropped. */
+    void scatter_q(std::vector<double> &rho) const
+    {
+        rho.assign((size_t)p.Nx * p.Ny, 0.0);
+        auto acc = [&](int I, int Jj, double v) {
+            if (I < 0 || Jj < 0 || I >= p.Nx || Jj >= p.Ny) return;
+            rho[(size_t)Jj * p.Nx + I] += v;
+        };
+        for (int k = 0; k < N; ++k)
+            charge_one(p.order, acc, v2(geo->origin[0], geo->origin[1]), v2(hr[0], hr[1]), Rp(k), Q[k]);
+        for (double &v : rho) v = 0.0 + v / (hr[0] * hr[1]);
+    }
+
+    /* PoissonSolver::computeField (PoissonSolver.cpp:85-201) followed by
+     * SCBoundaryCondition::resetOutsideDomain (SCBoundaryCondition.hh:164-196),
+     * as main.cpp:229-247 does.  gamma from the mean momentum, a plain sum
+     * over particles / N (Distribution.cpp:44-52, PoissonSolver.cpp:64-66);
+     * hx scaled by gamma (PoissonSolver.cpp:71). */
+    int poisson(double gamma_override)
+    {
+        double pm[2] = {0.0, 0.0};
+        for (int k = 0; k < N; ++k) { pm[0] += P[2 * k]; pm[1] += P[2 * k + 1]; }
+        pm[0] /= N; pm[1] /= N;
+        const double gamma = gamma_override > 0 ? gamma_override : std::sqrt(pm[0] * pm[0] + pm[1] * pm[1] + 1);
+        const double hr0 = hr[0] * gamma, hr1 = hr[1];
+        std::vector<uint8_t> flag, active;
+        poisson_classes(*geo, flag, active);
+        const size_t nn = (size_t)p.Nx * p.Ny;
+        std::vector<double> rho, rhs(nn), phi(nn);
+        const double cc = 1 / (EPS_0 * gamma);               /* fillRHS, :451 */
+        auto solve = [&]() {
+            for (size_t k = 0; k < nn; ++k) rhs[k] = rho[k] * cc;
+            return poisson_direct(p.Nx, p.Ny, hr0, hr1, flag.data(), active.data(), rhs.data(), phi.data());
+        };
+        auto ph = [&](int i, int j) { return (i < p.Nx && j < p.Ny) ? phi[(size_t)j * p.Nx + i] : 0.0; };
+        const std::vector<double> saved = R;                 /* saveR */
+        /* solve 1: Ey from rho at t + dt/2 (:154-166, fillFields case 1) */
+        drift(dt / 2);
+        scatter_q(rho);
+        drift(-dt / 2);
+        if (solve()) return -1;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i)
+                E.at(1, i, j) = gamma * ph(i, j) / hr1 - gamma * ph(i, j + 1) / hr1;
+        /* solve 2: Hz, bunch moved back half a cell (:167-178, case 2) */
+        for (int k = 0; k < N; ++k) R[2 * k] = R[2 * k] + (-0.5 * hr0 / gamma);
+        scatter_q(rho);
+        if (solve()) return -1;
+        const double betaGamma = std::sqrt(gamma * gamma - 1);
+        const double den = 2 * C_LIGHT * MU_0 * hr1;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) {
+                const double h = betaGamma * ph(i, j) / den - betaGamma * ph(i, j + 1) / den;
+                H.at(0, i, j) = h;
+                H.at(1, i, j) = h;
+            }
+        /* solve 3: Ex (:179-195, case 0) */
+        R = saved;                                           /* restoreR */
+        scatter_q(rho);
+        if (solve()) return -1;
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) E.at(0, i, j) = ph(i, j) / hr0 - ph(i + 1, j) / hr0;
+        /* resetOutsideDomain */
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i) {
+                const size_t id = (size_t)j * p.Nx + i;
+                if (geo->mask[id]) continue;
+                const int bc = geo->bmap[id];
+                if (bc == -1) { E.at(0, i, j) = 0.0; E.at(1, i, j) = 0.0; continue; }
+                const BCell &c = geo->bcells[bc];
+                if (c.lam[0] < SPACE_EPS) E.at(0, i, j) = 0.0;
+                if (c.lam[1] < SPACE_EPS) E.at(1, i, j) = 0.0;
+                if (c.area < MINAREA) { E.at(0, i, j) = 0.0; E.at(1, i, j) = 0.0; }
+            }
+        /* the window copy and exchanges of main.cpp:335-341 */
+        copy_to_window();
+        const int pos[2] = {ddim[0][0], ddim[1][0]};
+        exchange(OP_H, pos);
+        exchange(OP_EX, pos);
+        exchange(OP_EY, pos);
+        return 0;
+    }
+
     /* main.cpp:390-506, one iteration */
     void step()
     {
@@ -1930,4 +2120,33 @@ void orc_sim_diag(void *sp, double *out)
     out[5] = (double)(en * s->hr[0] * s->hr[1]);
 }
 
+
+int orc_poisson_classes(void *bp, uint8_t *flag, uint8_t *active)
+{
+    std::vector<uint8_t> f, a;
+    Bend *b = as<Bend>(bp);
+    poisson_classes(*b, f, a);
+    std::memcpy(flag, f.data(), f.size());
+    std::memcpy(active, a.data(), a.size());
+    return (int)(size_t)(b->lenBefore / b->hr[0]);
+}
+int orc_poisson_solve(int nx, int ny, double hx, double hy, const uint8_t *flag, const uint8_t *active,
+                      const double *b, double *phi)
+{
+    try {
+        return poisson_direct(nx, ny, hx, hy, flag, active, b, phi);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle poisson: %s\n", e.what());
+        return -1;
+    }
+}
+int orc_sim_poisson(void *s, double gamma)
+{
+    try {
+        return as<Sim>(s)->poisson(gamma);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle poisson: %s\n", e.what());
+        return -1;
+    }
+}
 } /* extern "C" */

@@ -97,6 +97,19 @@ int   orc_sim_set_field(void *s, int which, const double *in);
  * H/2, SC Hz, Ex, SC Ex, H/2, SC Hz, Ey, SC Ey with J as set. */
 int   orc_sim_field_steps(void *s, int nsteps);
 
+/* ---- Poisson initialisation (PoissonSolver.cpp:85-201, 678-766) ------ */
+/* Node classes of cutoffStencil on the bend's Nx x Ny grid (row-major);
+ * returns maxX = lengthStraightSection in cells. */
+int   orc_poisson_classes(void *bend, uint8_t *flag, uint8_t *active);
+/* Direct solve of the cutoffStencil system with spacings hx (already
+ * gamma-scaled), hy; identity rows give phi = b.  0 ok, -1 not SPD. */
+int   orc_poisson_solve(int nx, int ny, double hx, double hy, const uint8_t *flag,
+                        const uint8_t *active, const double *b, double *phi);
+/* After orc_sim_init: the three Poisson solves into the global E and H,
+ * resetOutsideDomain, then the window copy and exchanges.  gamma <= 0 takes
+ * it from the mean momentum. */
+int   orc_sim_poisson(void *s, double gamma);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,9 @@ def lib():
         L.orc_bend_is_inside.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_sim_create.restype = C.c_void_p
         L.orc_sim_create.argtypes = [C.POINTER(SimParams)]
+        L.orc_poisson_solve.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
+        L.orc_sim_poisson.argtypes = [C.c_void_p, C.c_double]
         _LIB = L
     return _LIB
 
@@ -136,6 +139,21 @@ def charge_density(order, box, origin, dx, R, Q):
     return rho
 
 
+def poisson_solve(hx, hy, flag, active, b):
+    """Direct solve of the cutoffStencil system (PoissonSolver.cpp:678-766);
+    flag, active, b are [ny, nx]; identity rows return phi = b."""
+    flag = np.ascontiguousarray(flag, np.uint8)
+    active = np.ascontiguousarray(active, np.uint8)
+    b = np.ascontiguousarray(b, np.float64)
+    ny, nx = b.shape
+    phi = np.zeros_like(b)
+    rc = lib().orc_poisson_solve(nx, ny, hx, hy, flag.ctypes.data, active.ctypes.data, b.ctypes.data,
+                                 phi.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle poisson solve failed (%d)" % rc)
+    return phi
+
+
 class Bend:
     """Bend geometry (Bend.cpp) on an Nx x Ny global vertex mesh."""
 
@@ -173,6 +191,13 @@ class Bend:
         lib().orc_bend_bcells(C.c_void_p(self.h), _ptr(ij, C.c_int), _ptr(data))
         return ij, data
 
+    def poisson_classes(self):
+        """(flag, active, maxX) of PoissonSolver::cutoffStencil on this grid, [Ny, Nx] uint8."""
+        f = np.zeros((self.Ny, self.Nx), np.uint8)
+        a = np.zeros((self.Ny, self.Nx), np.uint8)
+        mx = lib().orc_poisson_classes(C.c_void_p(self.h), _ptr(f, C.c_uint8), _ptr(a, C.c_uint8))
+        return f, a, mx
+
     def external_field(self, R):
         R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
         out = np.zeros(len(R))
@@ -204,6 +229,12 @@ class Sim:
         rc = lib().orc_sim_init(C.c_void_p(self.h), _ptr(R), _ptr(P), _ptr(Q), C.c_int(self.n))
         if rc != 0:
             raise RuntimeError("oracle init failed (%d)" % rc)
+
+    def poisson(self, gamma=0.0):
+        """PoissonSolver::computeField + resetOutsideDomain + window copy (after init)."""
+        rc = lib().orc_sim_poisson(C.c_void_p(self.h), C.c_double(gamma))
+        if rc != 0:
+            raise RuntimeError("oracle poisson failed (%d)" % rc)
 
     def step(self, n=1):
         rc = lib().orc_sim_step(C.c_void_p(self.h), C.c_int(n))
