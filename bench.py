@@ -19,8 +19,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_PART = {True: 48, False: 64}     # bytes per particle-step (FP32 split, FP64)
-B_CELL = {True: 48, False: 96}     # bytes per cell-update (FP32, FP64)
+# Algorithmic HBM bytes.  Per particle per push launch: the particle state
+# (int32 cell x/y, offset x/y, momentum x/y) read once and written once:
+# FP32 2*(8 + 16) = 48 B, FP64 2*(8 + 32) = 80 B.  Per cell per step: k_heh
+# reads Ex, Ey, Hz and the class byte and writes Ex', Hz'; k_ey reads Ey, Hz
+# and the class byte and writes Ey (currents only inside the bunch footprint):
+# FP32 13 + 8 + 9 + 4 = 34 B, FP64 25 + 16 + 17 + 8 = 66 B.
+B_PART = {True: 48, False: 80}
+B_CELL = {True: 34, False: 66}
 
 
 def run_mw(cfg, fp32, steps, warmup, R, P, Q, stat):
@@ -68,6 +74,42 @@ def cpu_baseline(seconds=12.0):
                        "%d steps in %.1f s on one host core" % (steps, t))
 
 
+def copy_probe(mib):
+    """One device-to-device hipMemcpy of mib MiB (the runtime's copyBuffer
+    kernel): reads and writes mib MiB, a known traffic for the HBM counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("copy probe: hipMalloc failed")
+    if hip.hipMemset(src, 1, nbytes) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("copy probe: hipMemset failed")
+    if hip.hipMemcpy(dst, src, nbytes, 3) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("copy probe: hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print("bench: copy probe %d MiB read + %d MiB written" % (mib, mib), file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_push launch from the committed rocprofv3 --pmc summary,
+    when it was measured on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_push_c2_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+    if kv.get("workload") != workload or "hbm_bytes_per_launch" not in kv:
+        return None
+    return float(kv["hbm_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +120,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB, to calibrate HBM counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the PIC step on one GPU only; --gpus %d is not supported" % a.gpus,
@@ -90,6 +134,8 @@ def main():
     if not _lib.has_gpu():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
     cfg = config(a.config)
     n = a.particles or cfg["n"]
     print("bench: config %s, %d particles, %dx%d cells, %d+%d steps" % (a.config, n, cfg["W"], cfg["H"], a.warmup,
@@ -114,6 +160,9 @@ def main():
     sweep_ms = tim32["sweep_ms"]
     achieved = n * B_PART[True] / (push_launch_ms * 1e-3) / 1e9
     frac_step = (cells * B_CELL[True] + n * B_PART[True]) * a.steps / t32 / 1e9 / HBM_PEAK_GBS
+    workload = ("%s: %dx%d window dx=%g um, %.0f deg bend R=%g m, gamma %g, %g nC, %d particles, shape S%d, "
+                "sort every %d" % (a.config, cfg["W"], cfg["H"], cfg["dx"] * 1e6, cfg["phiDeg"], cfg["R"],
+                                   cfg["gamma"], cfg["charge_nC"], n, cfg["order"], cfg["sortEvery"]))
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()
@@ -130,15 +179,11 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (split int32+fp32 positions)",
         "data": "synthetic seeded Gaussian bunch (Philox), analytic bend chamber",
-        "config": {"workload": "%s: %dx%d window dx=%g um, %.0f deg bend R=%g m, gamma %g, %g nC, %d particles, "
-                               "shape S%d, sort every %d" % (a.config, cfg["W"], cfg["H"], cfg["dx"] * 1e6,
-                                                              cfg["phiDeg"], cfg["R"], cfg["gamma"],
-                                                              cfg["charge_nC"], n, cfg["order"], cfg["sortEvery"]),
-                   "cells": cells, "particles": n},
+        "config": {"workload": workload, "cells": cells, "particles": n},
         "cell_updates_per_s": cu32,
         "step_roofline_frac": frac_step,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "k_push (push + deposit, %d B per particle per launch)" % B_PART[True],
                      "launch_ms": push_launch_ms, "sweep_ms_per_step": sweep_ms},
         "fp64": rec64,

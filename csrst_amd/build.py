@@ -56,7 +56,23 @@ def build(verbose=True, jobs=4):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose=True):
+    """The C++ front end's command-line driver (host/csrst_main.cpp), linked
+    against libcsrst.so in this directory."""
+    exe = os.path.join(HERE, "csrst")
+    tmp = exe + ".tmp%d" % os.getpid()
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "csrst_main.cpp"), "-o", tmp, "-L" + HERE, "-lcsrst",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, exe)
+    return exe
 
 
 if __name__ == "__main__":

@@ -1,0 +1,391 @@
+// csrst.hpp -- C++ front end of libcsrst (include/csrst_hip.h) with the
+// reference's class interfaces for the particle-in-cell time step:
+//   Commands      the command line of main.cpp:111-136
+//   Distribution  the bunch generator of Distribution.cpp:55-130
+//   Engine        AllocateFields + Bend::Init + the window (main.cpp:168-217);
+//                 owns the csrst_ctx
+//   Bend          geometry queries (Bend.hh:373-485)
+//   PartBunch     particle set-up and statistics (PartBunch.hh:35-206)
+//   TETMSolver    updateField_firstStep / secondStep (TETMSolver.hh:38-103)
+//   PoissonSolver computeField (PoissonSolver.hh:47-66)
+// Errors are thrown as csrst::Error; the reference aborts (PAssert) or
+// returns EXIT_FAILURE from main instead.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "csrst_hip.h"
+
+namespace csrst {
+
+class Error : public std::runtime_error {
+public:
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const void *handle = nullptr)
+{
+    if (rc != CSRST_OK) {
+        const char *m = csrst_last_error(handle);
+        throw Error(rc, m ? m : "csrst error");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// main.cpp:111-136 (TCLAP flags, same names, short forms and defaults), plus
+// the choices the reference makes at compile time (IntOp_t, defs.hh:55-57).
+struct Commands {
+    std::string restart;
+    int startstep = 0;
+    int numsteps = 10;
+    int numgridpx = 100;
+    int numgridpy = 100;
+    int plotmodulus = 100000;
+    int statmodulus = 100000;
+    int linemodulus = 100000;
+    int reballance = 100000;
+    double phi = 0.0;
+    double width = 0.10;
+    double length = 0.10;
+    double lengthbefore = 0.10;
+    double Ekin = 250.0;
+    double Bz = 0.10;
+    double Qtotal = -1.0;
+    double mean_Rx = 0.05;
+    double mean_Ry = 0.0;
+    double sigma_x = 0.01;
+    double sigma_p = 1.0;
+    int Np = 50000;
+    std::string h5file = "Output.h5part";
+    bool no_initials = false;
+    bool debug = false;
+    // compile-time in the reference
+    int order = 0;          // --order: 0 IntCIC, 1 IntS1, 2 IntS2
+    bool s2bug = true;      // --no-s2bug: gather IntS2 at (I+2, J+2) instead of ScatterGatherImpl.hh:467
+    // additions of this front end
+    std::string bunch;      // --bunch FILE: n x 5 doubles (x, y, px, py, q) instead of the generator
+    std::string wake;       // --wake FILE: write the CSR wake samples at the end
+    double poissontol = 1e-12;   // main.cpp:94 relativeConvergenceTolerance
+    int poissonmaxit = 500;      // main.cpp:93 maxNumPoissonIterations
+
+    static Commands parse(int argc, char **argv);
+    csrst_params params() const
+    {
+        csrst_params p;
+        p.Nx = numgridpx;
+        p.Ny = numgridpy;
+        p.lenBefore = lengthbefore;
+        p.lenAfter = length;
+        p.width = width;
+        p.phiDeg = phi;
+        p.ekinMeV = Ekin;
+        p.bz = Bz;
+        p.order = order;
+        p.s2bug = s2bug ? 1 : 0;
+        p.statModulus = statmodulus;
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Distribution.cpp:55-130: two-dimensional Halton points (bases 2 and 3, one
+// stream for positions and one for momenta) through the Box-Muller map of
+// gaussian2DSample (Distribution.cpp:146-157); R = mean + x, P = (p0, 0) + p
+// with gamma = Ekin / (m_e 1e3) + 1; Q = -Qtotal 1e-9 / Np each; R alone is
+// then sorted by Utils::XFirst (utils.hh:116-140), which re-pairs positions
+// and momenta exactly as the reference does.
+struct Bunch {
+    std::vector<double> R, P, Q;   // R, P: 2 per particle
+    size_t size() const { return Q.size(); }
+};
+
+class Distribution {
+public:
+    // radical inverse of k in base b (the Halton coordinate)
+    static double radical_inverse(unsigned long long k, unsigned b)
+    {
+        double f = 1.0, r = 0.0;
+        while (k) {
+            f /= b;
+            r += f * (double)(k % b);
+            k /= b;
+        }
+        return r;
+    }
+    static Bunch generate(const Commands &c, const double hr[2], const double origin[2]);
+};
+
+// ---------------------------------------------------------------------------
+struct BoundaryCell {       // BoundaryCell.hh:28-33
+    int i, j;
+    double lambda[2];
+    double area;
+    double lambdaToNeighbours[2];
+};
+
+class Engine {
+public:
+    explicit Engine(const Commands &cmd) : cmd_(cmd)
+    {
+        const csrst_params p = cmd.params();
+        check(csrst_create(&p, &ctx_));
+        check(csrst_geometry(ctx_, geo_), ctx_);
+    }
+    ~Engine() { csrst_destroy(ctx_); }
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    csrst_ctx *ctx() const { return ctx_; }
+    const Commands &commands() const { return cmd_; }
+    double dt() const { return geo_[4]; }
+    const double *hr() const { return geo_; }
+    const double *origin() const { return geo_ + 2; }
+    double radius() const { return geo_[5]; }
+    double phi() const { return geo_[6]; }
+
+    // main.cpp:494-501 (moveMesh) with the wake sample and J reset before it
+    std::array<int, 2> moveMesh()
+    {
+        int mi = 0, mj = 0;
+        check(csrst_move_window(ctx_, &mi, &mj), ctx_);
+        return {mi, mj};
+    }
+    // IntegrateElongCommand accumulator (IntegrateElong.cpp:38-125)
+    std::vector<double> wake() const
+    {
+        int n = 0;
+        check(csrst_wake(ctx_, nullptr, 0, &n), ctx_);
+        std::vector<double> w(n > 0 ? n : 1);
+        check(csrst_wake(ctx_, w.data(), (int)w.size(), &n), ctx_);
+        w.resize(n);
+        return w;
+    }
+    // dt, window origin (2), window position (2), W, H, steps, wake samples
+    std::array<double, 9> info() const
+    {
+        std::array<double, 9> a{};
+        check(csrst_info(ctx_, a.data()), ctx_);
+        return a;
+    }
+    // which: 0 E, 1 H (global); 2 E, 3 H, 4 J (window); 5 J (global)
+    std::vector<double> field(int which) const
+    {
+        int n = 0;
+        check(csrst_download_field(ctx_, which, nullptr, 0, &n), ctx_);
+        std::vector<double> f(n);
+        check(csrst_download_field(ctx_, which, f.data(), n, &n), ctx_);
+        return f;
+    }
+
+private:
+    Commands cmd_;
+    csrst_ctx *ctx_ = nullptr;
+    double geo_[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+class Bend {
+public:
+    explicit Bend(const Engine &e) : e_(e) {}
+    double getRadius() const { return e_.radius(); }
+    double getAngle() const { return e_.phi(); }
+    std::array<double, 2> getMeshSpacing() const { return {e_.hr()[0], e_.hr()[1]}; }
+    std::array<double, 2> getOrigin() const { return {e_.origin()[0], e_.origin()[1]}; }
+    // Bend.hh:288-291
+    size_t getLengthStraightSection() const { return (size_t)(e_.commands().lengthbefore / e_.hr()[0]); }
+    std::vector<uint8_t> getInsideMask() const
+    {
+        std::vector<uint8_t> m((size_t)e_.commands().numgridpx * e_.commands().numgridpy);
+        int n = 0;
+        check(csrst_boundary_cells(e_.ctx(), m.data(), nullptr, nullptr, 0, &n), e_.ctx());
+        return m;
+    }
+    std::vector<BoundaryCell> getBoundaryCells() const
+    {
+        int n = 0;
+        check(csrst_boundary_cells(e_.ctx(), nullptr, nullptr, nullptr, 0, &n), e_.ctx());
+        std::vector<int> ij(2 * (size_t)n);
+        std::vector<double> d(5 * (size_t)n);
+        check(csrst_boundary_cells(e_.ctx(), nullptr, ij.data(), d.data(), n, &n), e_.ctx());
+        std::vector<BoundaryCell> out(n);
+        for (int k = 0; k < n; ++k)
+            out[k] = BoundaryCell{ij[2 * k], ij[2 * k + 1], {d[5 * k], d[5 * k + 1]}, d[5 * k + 2],
+                                  {d[5 * k + 3], d[5 * k + 4]}};
+        return out;
+    }
+
+private:
+    const Engine &e_;
+};
+
+class PartBunch {
+public:
+    // PartBunch(...) + makeReady(dt/2) + the window set-up of main.cpp:199-347
+    PartBunch(Engine &e, const Bunch &b) : e_(e), n_(b.size())
+    {
+        check(csrst_init_bunch(e.ctx(), b.R.data(), b.P.data(), b.Q.data(), (int)n_), e.ctx());
+    }
+    size_t getTotalNum() const { return n_; }
+    void get_rmean(double out[2]) const { diag(); out[0] = d_[0]; out[1] = d_[1]; }
+    void get_pmean(double out[2]) const { diag(); out[0] = d_[2]; out[1] = d_[3]; }
+    double get_qtotal() const { diag(); return d_[4]; }
+    double fieldEnergy() const { diag(); return d_[5]; }
+    void getParticles(std::vector<double> &R, std::vector<double> &P, std::vector<double> &oldR) const
+    {
+        R.resize(2 * n_); P.resize(2 * n_); oldR.resize(2 * n_);
+        check(csrst_download_particles(e_.ctx(), R.data(), P.data(), oldR.data(), (int)n_), e_.ctx());
+    }
+
+private:
+    void diag() const { check(csrst_diag(e_.ctx(), d_), e_.ctx()); }
+    Engine &e_;
+    size_t n_;
+    mutable double d_[6] = {0, 0, 0, 0, 0, 0};
+};
+
+class TETMSolver {
+public:
+    explicit TETMSolver(Engine &e) : e_(e) {}
+    void updateField_firstStep() { check(csrst_first_step(e_.ctx()), e_.ctx()); }
+    void updateField_secondStep() { check(csrst_second_step(e_.ctx()), e_.ctx()); }
+
+private:
+    Engine &e_;
+};
+
+class PoissonSolver {
+public:
+    explicit PoissonSolver(Engine &e) : e_(e) {}
+    // three solves, fillFields, resetOutsideDomain, window copy
+    void computeField(double tol, int maxIterations)
+    {
+        check(csrst_poisson_compute_field(e_.ctx(), tol, maxIterations, cycles, resid), e_.ctx());
+    }
+    int cycles[3] = {0, 0, 0};
+    double resid[3] = {0, 0, 0};
+
+private:
+    Engine &e_;
+};
+
+// ---------------------------------------------------------------------------
+// inline definitions
+
+// Flags as TCLAP accepts them: "-x 64", "--numgridpx 64", "--numgridpx=64";
+// switches -I/--no_initials and -d/--debug.  An unknown flag or a bad value
+// throws (the reference prints the TCLAP error and returns EXIT_FAILURE).
+inline Commands Commands::parse(int argc, char **argv)
+{
+    Commands c;
+    struct Opt {
+        const char *shortName, *longName;
+        char kind;   // i int, d double, s string, b switch
+        void *dst;
+    };
+    const Opt opts[] = {
+        {"r", "restart", 's', &c.restart},         {"n", "startstep", 'i', &c.startstep},
+        {"N", "numsteps", 'i', &c.numsteps},       {"x", "numgridpx", 'i', &c.numgridpx},
+        {"y", "numgridpy", 'i', &c.numgridpy},     {"m", "plotmodulus", 'i', &c.plotmodulus},
+        {"", "statmodulus", 'i', &c.statmodulus},  {"", "linemodulus", 'i', &c.linemodulus},
+        {"", "reballance", 'i', &c.reballance},    {"P", "phi", 'd', &c.phi},
+        {"w", "width", 'd', &c.width},             {"l", "length", 'd', &c.length},
+        {"", "lengthbefore", 'd', &c.lengthbefore}, {"E", "Ekin", 'd', &c.Ekin},
+        {"B", "Bz", 'd', &c.Bz},                   {"Q", "Qtotal", 'd', &c.Qtotal},
+        {"", "mean_Rx", 'd', &c.mean_Rx},          {"", "mean_Ry", 'd', &c.mean_Ry},
+        {"s", "sigma_x", 'd', &c.sigma_x},         {"S", "sigma_p", 'd', &c.sigma_p},
+        {"", "Np", 'i', &c.Np},                    {"", "h5file", 's', &c.h5file},
+        {"I", "no_initials", 'b', &c.no_initials}, {"d", "debug", 'b', &c.debug},
+        {"", "order", 'i', &c.order},              {"", "bunch", 's', &c.bunch},
+        {"", "wake", 's', &c.wake},                {"", "poissontol", 'd', &c.poissontol},
+        {"", "poissonmaxit", 'i', &c.poissonmaxit},
+    };
+    for (int k = 1; k < argc; ++k) {
+        std::string a = argv[k], val;
+        bool hasVal = false;
+        const Opt *o = nullptr;
+        if (a == "--no-s2bug") { c.s2bug = false; continue; }
+        if (a.size() > 2 && a[0] == '-' && a[1] == '-') {
+            std::string name = a.substr(2);
+            const size_t eq = name.find('=');
+            if (eq != std::string::npos) { val = name.substr(eq + 1); name = name.substr(0, eq); hasVal = true; }
+            for (const Opt &x : opts) if (name == x.longName) o = &x;
+        } else if (a.size() == 2 && a[0] == '-') {
+            for (const Opt &x : opts) if (a.substr(1) == x.shortName) o = &x;
+        }
+        if (!o) throw Error(CSRST_EINVAL, "unknown argument: " + a);
+        if (o->kind == 'b') { *(bool *)o->dst = true; continue; }
+        if (!hasVal) {
+            if (k + 1 >= argc) throw Error(CSRST_EINVAL, "missing value for " + a);
+            val = argv[++k];
+        }
+        char *end = nullptr;
+        if (o->kind == 'i') {
+            const long v = std::strtol(val.c_str(), &end, 10);
+            if (!end || *end) throw Error(CSRST_EINVAL, "not an integer for " + a + ": " + val);
+            *(int *)o->dst = (int)v;
+        } else if (o->kind == 'd') {
+            const double v = std::strtod(val.c_str(), &end);
+            if (!end || *end) throw Error(CSRST_EINVAL, "not a number for " + a + ": " + val);
+            *(double *)o->dst = v;
+        } else {
+            *(std::string *)o->dst = val;
+        }
+    }
+    if (c.order < 0 || c.order > 2) throw Error(CSRST_EINVAL, "--order must be 0, 1 or 2");
+    if (!c.restart.empty()) throw Error(CSRST_EINVAL, "--restart (H5Part restart files) is not supported");
+    return c;
+}
+
+inline Bunch Distribution::generate(const Commands &c, const double hr[2], const double origin[2])
+{
+    const int n = c.Np;
+    if (n <= 0) throw Error(CSRST_EINVAL, "--Np must be positive");
+    const double twoPi = 2.0 * 3.14159265358979323846;
+    const double mass = 0.51099906e-03 * 1e3;              // Distribution.cpp:26, MeV
+    const double gamma = c.Ekin / mass + 1;
+    const double meanP = std::sqrt(gamma * gamma - 1);
+    const double qEach = (-c.Qtotal * 1e-9) / n;
+    Bunch b;
+    b.R.resize(2 * (size_t)n);
+    b.P.resize(2 * (size_t)n);
+    b.Q.assign(n, qEach);
+    auto sample = [&](unsigned long long k, double sigma, double out[2]) {
+        const double v0 = radical_inverse(k, 2), v1 = radical_inverse(k, 3);
+        const double dist = sigma * std::sqrt(-2 * std::log(v0));
+        const double angle = twoPi * v1;
+        out[0] = dist * std::cos(angle);
+        out[1] = dist * std::sin(angle);
+    };
+    for (int i = 0; i < n; ++i) {
+        double x[2], p[2];
+        sample((unsigned long long)i + 1, c.sigma_x, x);   // one Halton stream each
+        sample((unsigned long long)i + 1, c.sigma_p, p);
+        b.R[2 * i] = c.mean_Rx + x[0];
+        b.R[2 * i + 1] = c.mean_Ry + x[1];
+        b.P[2 * i] = meanP + p[0];
+        b.P[2 * i + 1] = 0.0 + p[1];
+    }
+    // Utils::XFirst (utils.hh:116-140), applied to R only: by y cell, then by
+    // x value.  XFirst returns true for equal x (not a strict weak order);
+    // equal x never occurs for these samples, so ">" orders the same.
+    std::vector<std::array<double, 2>> r(n);
+    for (int i = 0; i < n; ++i) r[i] = {b.R[2 * i], b.R[2 * i + 1]};
+    std::sort(r.begin(), r.end(), [&](const std::array<double, 2> &u, const std::array<double, 2> &v) {
+        const int J1 = (int)((u[1] - origin[1]) / hr[1]), J2 = (int)((v[1] - origin[1]) / hr[1]);
+        if (J2 > J1) return true;
+        if (J2 < J1) return false;
+        return v[0] > u[0];
+    });
+    for (int i = 0; i < n; ++i) { b.R[2 * i] = r[i][0]; b.R[2 * i + 1] = r[i][1]; }
+    return b;
+}
+
+}  // namespace csrst
