@@ -25,7 +25,10 @@ SYMBOLS = [
     "csrst_mw_upload_fields",
     "csrst_mw_download_fields", "csrst_mw_download_particles", "csrst_mw_info", "csrst_mw_wake",
     "csrst_mw_timing",
-    "csrst_poisson_solve", "csrst_poisson_compute_field",
+    "csrst_poisson_solve", "csrst_poisson_compute_field", "csrst_mw_poisson",
+    "csrst_warmstart_create", "csrst_warmstart_destroy", "csrst_warmstart_num_params",
+    "csrst_warmstart_get_weights", "csrst_warmstart_set_weights", "csrst_warmstart_infer",
+    "csrst_warmstart_gradients", "csrst_warmstart_train_step",
 ]
 
 

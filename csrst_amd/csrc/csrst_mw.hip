@@ -38,6 +38,7 @@
 #include "bend_geom.h"
 #include "hip_util.h"
 #include "pic_math.h"
+#include "poisson_mg.h"
 
 using hu::DBuf;
 using hu::Error;
@@ -235,6 +236,14 @@ struct Split {
 template <class T>
 __device__ __forceinline__ T absv(T v) { return v < T(0) ? -v : v; }
 
+// component-wise select: a select of whole structs can be lowered to a select
+// of stack addresses, which puts the operands in scratch memory
+template <class T>
+__device__ __forceinline__ vec2<T> sel(bool c, vec2<T> a, vec2<T> b)
+{
+    return pm::mk(c ? a.x : b.x, c ? a.y : b.y);
+}
+
 template <class T>
 __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bool dual)
 {
@@ -246,20 +255,23 @@ __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bo
     const T eps = T(1e-6);
     // the (at most two) interior crossings
     T fa = T(2), fb = T(2);
-    vec2<T> sa = o, sb = o;
+    T sax = o.x, say = o.y, sbx = o.x, sby = o.y;
     if (dI != 0) {
         const T xb = (T)(dI > 0 ? If : Ii) + sh;
         fa = (xb - o.x) / D.x;
-        sa = pm::mk(xb, o.y + fa * D.y);
+        sax = xb;
+        say = o.y + fa * D.y;
     }
     if (dJ != 0) {
         const T yb = (T)(dJ > 0 ? Jf : Ji) + sh;
         fb = (yb - o.y) / D.y;
-        sb = pm::mk(o.x + fb * D.x, yb);
+        sbx = o.x + fb * D.x;
+        sby = yb;
     }
+    const vec2<T> sa = pm::mk(sax, say), sb = pm::mk(sbx, sby);
     const bool swp = fb < fa;
     const T f1 = swp ? fb : fa, f2 = swp ? fa : fb;
-    const vec2<T> s1 = swp ? sb : sa, s2 = swp ? sa : sb;
+    const vec2<T> s1 = sel(swp, sb, sa), s2 = sel(swp, sa, sb);
     // duplicates within 1e-6 in fraction merge into the earlier entry
     bool v1 = f1 < T(2), v2 = f2 < T(2);
     v1 = v1 && !(absv(f1) < eps);
@@ -270,14 +282,14 @@ __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bo
     // fixed slots: 0 start, then kept crossings, then the end
     c.f[0] = T(0); c.s[0] = o;
     c.f[1] = v1 ? f1 : (v2 ? f2 : T(1));
-    c.s[1] = v1 ? s1 : (v2 ? s2 : e);
+    c.s[1] = sel(v1, s1, sel(v2, s2, e));
     c.f[2] = (v1 && v2) ? f2 : T(1);
-    c.s[2] = (v1 && v2) ? s2 : e;
+    c.s[2] = sel(v1 && v2, s2, e);
     c.f[3] = T(1); c.s[3] = e;
     const int n = 2 + (v1 ? 1 : 0) + (v2 ? 1 : 0) - (endMerged ? 1 : 0);
     // a merged end: the last kept crossing takes the end position (its fraction stays)
-    c.s[1] = (endMerged && n == 2) ? e : c.s[1];
-    c.s[2] = (endMerged && n == 3) ? e : c.s[2];
+    c.s[1] = sel(endMerged && n == 2, e, c.s[1]);
+    c.s[2] = sel(endMerged && n == 3, e, c.s[2]);
     c.n = n;
     return (dI >= -1 && dI <= 1 && dJ >= -1 && dJ <= 1);
 }
@@ -285,32 +297,63 @@ __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bo
 constexpr int kPPT = 8;              // particles per thread in one workgroup chunk
 constexpr int kChunk = kB * kPPT;    // 2048 particles per workgroup
 constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
+constexpr int kFW = 48, kFH = 24;    // LDS field tile for the gathers (nodes)
 
 template <class T, int ORDER, int COMP>
 __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
     __shared__ T acc[kTH * kTW];
-    __shared__ int sbox[4];          // chunk box of old cells, then written-node box
-    __shared__ int wbox[4];
+    __shared__ T ftile[3][kFH * kFW];   // Ex, Ey, Hz around the chunk
+    __shared__ int sbox[4];          // chunk box of old cells
+    __shared__ int wbox[4];          // written-node box of the accumulation tile
     const int tid = threadIdx.x, lane = tid & 63;
     const int base = blockIdx.x * kChunk;
     const int wx0 = ws->wx0, wy0 = ws->wy0, rx = ws->rx, ry = ws->ry;
     const vec2<T> p0 = pm::mk((T)ws->p0[0], (T)ws->p0[1]);
     const int W = f.W, H = f.H;
     T *J = COMP == 0 ? f.jx : f.jy;
-    // ---- tile anchor: minimum old cell of the chunk
-    int mnx = INT_MAX, mny = INT_MAX;
+    // ---- chunk box of old cells: anchor of the accumulation tile and extent
+    // of the field tile
+    int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
     for (int it = 0; it < kPPT; ++it) {
         const int k = base + it * kB + tid;
-        if (k < p.n) { mnx = min(mnx, p.gx[k]); mny = min(mny, p.gy[k]); }
+        if (k < p.n) {
+            const int cx = p.gx[k], cy = p.gy[k];
+            mnx = min(mnx, cx); mny = min(mny, cy); mxx = max(mxx, cx); mxy = max(mxy, cy);
+        }
     }
     mnx = hu::wave_min(mnx); mny = hu::wave_min(mny);
-    if (tid == 0) { sbox[0] = INT_MAX; sbox[1] = INT_MAX; wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN; }
+    mxx = hu::wave_max(mxx); mxy = hu::wave_max(mxy);
+    if (tid == 0) {
+        sbox[0] = INT_MAX; sbox[1] = INT_MAX; sbox[2] = INT_MIN; sbox[3] = INT_MIN;
+        wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN;
+    }
     for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
     __syncthreads();
-    if (lane == 0) { atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny); }
+    if (lane == 0) {
+        atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny);
+        atomicMax(&sbox[2], mxx); atomicMax(&sbox[3], mxy);
+    }
     __syncthreads();
     const int ax = sbox[0] - 2, ay = sbox[1] - 2;   // LDS node (0,0) = global node (ax, ay)
+    // Field tile: the gathers at the half-drifted positions read nodes within
+    // [-3, +3] cells of a particle's old cell (half drift <= 1/2 cell, S2
+    // stencils span [-2, +2] about the new cell), so old-cell box -4 .. +3.
+    // A chunk whose box does not fit (sparse tails) gathers from HBM.
+    const int tx0 = sbox[0] - 4, ty0 = sbox[1] - 4;
+    const int tw = sbox[2] - sbox[0] + 8, th = sbox[3] - sbox[1] + 8;
+    const bool tiled = sbox[0] <= sbox[2] && tw <= kFW && th <= kFH && tx0 - wx0 >= -G &&
+                       tx0 + tw - 1 - wx0 <= W - 1 + G && ty0 - wy0 >= -G && ty0 + th - 1 - wy0 <= H - 1 + G;
+    if (tiled) {
+        for (int q = tid; q < tw * th; q += kB) {
+            const int li = q % tw, lj = q / tw;
+            const size_t o = (size_t)ringy(ty0 + lj - wy0, ry, f.Hp) * f.Wp + ringx(tx0 + li - wx0, rx, f.Wp);
+            ftile[0][lj * kFW + li] = f.ex[o];
+            ftile[1][lj * kFW + li] = f.ey[o];
+            ftile[2][lj * kFW + li] = f.hz[o];
+        }
+        __syncthreads();
+    }
     int err = 0;
     int bminx = INT_MAX, bminy = INT_MAX, bmaxx = INT_MIN, bmaxy = INT_MIN;
     int tminx = INT_MAX, tminy = INT_MAX, tmaxx = INT_MIN, tmaxy = INT_MIN;
@@ -362,7 +405,11 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             // ring offsets of the stencil rows/columns computed once per field
             const vec2<T> pp = pm::mk(fx, fy);
             constexpr int N = ORDER + 2;
-            auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
+            auto gat = [&](int fi, const T *plane, const pm::Stencil<T> &st) -> T {
+                if (tiled) {
+                    const T *tp = &ftile[fi][(gy + (int)st.J0 - ty0) * kFW + (gx + (int)st.I0 - tx0)];
+                    return pm::gather_n<N>(st, false, [&](int a, int b) { return tp[b * kFW + a]; });
+                }
                 int col[N];
                 size_t rowb[N];
 #pragma unroll
@@ -374,9 +421,9 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
                 }
                 return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
             };
-            const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
-            const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
-            const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
+            const T ex = gat(0, f.ex, pm::edge_stencil(ORDER, 0, pp));
+            const T ey = gat(1, f.ey, pm::edge_stencil(ORDER, 1, pp));
+            const T hf = gat(2, f.hz, pm::cell_stencil(ORDER, pp));
             const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
             const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
             const T bext = (T)geom::external_bz(g.bend, X, Y);
@@ -400,8 +447,8 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             pm::SegCurrent<T> sc;
             {
                 // slot selects instead of xs.s[s] keep the list in registers
-                const vec2<T> xi = s == 0 ? xs.s[0] : (s == 1 ? xs.s[1] : xs.s[2]);
-                const vec2<T> xf = s == 0 ? xs.s[1] : (s == 1 ? xs.s[2] : xs.s[3]);
+                const vec2<T> xi = sel(s == 0, xs.s[0], sel(s == 1, xs.s[1], xs.s[2]));
+                const vec2<T> xf = sel(s == 0, xs.s[1], sel(s == 1, xs.s[2], xs.s[3]));
                 const T fi = s == 0 ? xs.f[0] : (s == 1 ? xs.f[1] : xs.f[2]);
                 const T ff = s == 0 ? xs.f[1] : (s == 1 ? xs.f[2] : xs.f[3]);
                 if (ORDER == 0) {
@@ -501,9 +548,9 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     }
     __syncthreads();
     if (wbox[0] == INT_MAX) return;
-    const int tw = wbox[2] - wbox[0] + 1, th = wbox[3] - wbox[1] + 1;
-    for (int q = tid; q < tw * th; q += kB) {
-        const int li = wbox[0] + q % tw, lj = wbox[1] + q / tw;
+    const int ww = wbox[2] - wbox[0] + 1, wh = wbox[3] - wbox[1] + 1;
+    for (int q = tid; q < ww * wh; q += kB) {
+        const int li = wbox[0] + q % ww, lj = wbox[1] + q / ww;
         const T v = acc[lj * kTW + li];
         if (v == T(0)) continue;
         const int i = ax + li - wx0, j = ay + lj - wy0;
@@ -697,6 +744,105 @@ __global__ void k_fresh(Fields<T> f, Geo g, const WinState *ws)
     const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
     f.ex[q] = 0; f.ey[q] = 0; f.hz[q] = 0; f.jx[q] = 0; f.jy[q] = 0; f.ex2[q] = 0; f.hz2[q] = 0;
     cell_geometry(g, f, ws->wx0 + i, ws->wy0 + j, q);
+}
+
+// ---------------------------------------------------------------------------
+// Poisson start on the window grid (PoissonSolver.cpp:85-201 with the
+// window in place of the global grid; window edges are Dirichlet).
+
+// Vertex charge (getChargeDensityImpl) at the particles' positions moved by
+// drift * P / gamma (metres) and by shiftx (metres, x only), in window node
+// units; FP64 atomics into rho (W x H nodes, logical order).
+template <class T, int ORDER>
+__global__ void k_pois_scatter(Parts<T> p, Geo g, const WinState *ws, double q, double drift, double shiftx,
+                               double *rho, int W, int H)
+{
+    constexpr int N = ORDER + 2;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.n) return;
+    const double Px = ws->p0[0] + (double)p.px[k], Py = ws->p0[1] + (double)p.py[k];
+    const double gi = (double)pm::q_rsqrt((float)(1 + (Px * Px + Py * Py)));
+    const double u = (double)(p.gx[k] - ws->wx0) + (double)p.fx[k] + (drift * Px * gi + shiftx) / g.h[0];
+    const double v = (double)(p.gy[k] - ws->wy0) + (double)p.fy[k] + (drift * Py * gi) / g.h[1];
+    double wx[N], wy[N];
+    int I0, J0;
+    pm::charge_weights<ORDER>(pm::mk(u, v), q, wx, wy, I0, J0);
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            const int I = I0 + a, J = J0 + b;
+            if (I < 0 || J < 0 || I >= W || J >= H) continue;
+            unsafeAtomicAdd(&rho[(size_t)J * W + I], pm::charge_w<ORDER>(q, wx[a], wy[b]));
+        }
+}
+
+// cutoffStencil node classes (PoissonSolver.cpp:678-766) from the window's
+// cell classes: flagged = bulk, or cut with area > 0.5; a stencil row if also
+// left of the straight-section end; window edge nodes are fixed at zero.
+template <class T>
+__global__ void k_pois_classes(Fields<T> f, const WinState *ws, long long maxX, uint8_t *cls)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= f.W || j >= f.H) return;
+    const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
+    const uint8_t c = f.cls[q];
+    const bool flag = c == 1 || (c == 2 && f.cut[2 * f.plane + q] > T(0.5));
+    const bool edge = i == 0 || j == 0 || i == f.W - 1 || j == f.H - 1;
+    cls[(size_t)j * f.W + i] =
+        edge || !flag ? pmg::kFixed : ((long long)ws->wx0 + i < maxX ? pmg::kUnknown : pmg::kKnown);
+}
+
+// fillFields (PoissonSolver.cpp:462-507) into the window planes; comp 0 Ex,
+// 1 Ey, 2 Hz (the window keeps Hz = Hzx + Hzy, each half the difference).
+template <class T>
+__global__ void k_pois_fill(Fields<T> f, const WinState *ws, const double *phi, int comp, double scale, double den)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= f.W || j >= f.H) return;
+    const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
+    const size_t k = (size_t)j * f.W + i;
+    if (comp == 0) {
+        const double nb = i + 1 < f.W ? phi[k + 1] : 0.0;
+        f.ex[q] = (T)(phi[k] / den - nb / den);
+    } else {
+        const double nb = j + 1 < f.H ? phi[k + f.W] : 0.0;
+        const double v = scale * phi[k] / den - scale * nb / den;
+        if (comp == 1) f.ey[q] = (T)v;
+        else f.hz[q] = (T)(v + v);
+    }
+}
+
+// resetOutsideDomain (SCBoundaryCondition.hh:164-196) on the window.
+template <class T>
+__global__ void k_pois_reset(Fields<T> f, const WinState *ws)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= f.W || j >= f.H) return;
+    const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
+    const uint8_t c = f.cls[q];
+    if (c == 1) return;
+    if (c == 0) { f.ex[q] = T(0); f.ey[q] = T(0); return; }
+    if (f.cut[q] < T(pm::kSpaceEps)) f.ex[q] = T(0);
+    if (f.cut[f.plane + q] < T(pm::kSpaceEps)) f.ey[q] = T(0);
+    if (f.cut[2 * f.plane + q] < T(pm::kMinArea)) { f.ex[q] = T(0); f.ey[q] = T(0); }
+}
+
+__global__ void k_pois_rhs(const double *rho, double *b, size_t n, double hh, double cc)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = (0.0 + rho[k] / hh) * cc;
+}
+
+__global__ void k_pois_shift_guess(const double *phi, double *out, int nx, int ny, double tau)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)nx * ny) return;
+    const int i = (int)(k % nx);
+    const double v = phi[k];
+    const double t = tau > 0 ? (tau > 1 ? 1.0 : tau) : (-tau > 1 ? 1.0 : -tau);
+    const double nb = tau > 0 ? (i + 1 < nx ? phi[k + 1] : 0.0) : (i > 0 ? phi[k - 1] : 0.0);
+    out[k] = v + t * nb - t * v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,6 +1223,75 @@ void check_state(csrst_mw *m)
 
 }  // namespace
 
+template <class T, int ORDER>
+void mw_scatter(csrst_mw *m, double drift, double shiftx, double *rho)
+{
+    mw::k_pois_scatter<T, ORDER><<<(m->n + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(
+        m->parts<T>(m->partBuf), m->geo, m->ws.p, m->q, drift, shiftx, rho, m->W, m->H);
+}
+
+// PoissonSolver::computeField (PoissonSolver.cpp:85-201) on the window, then
+// resetOutsideDomain.  gamma from the mean momentum; hx scaled by gamma.
+template <class T>
+void mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid)
+{
+    const int W = m->W, H = m->H;
+    const size_t nn = (size_t)W * H;
+    mw::k_moments<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->geo, m->ws.p, m->partial.p);
+    mw::k_moments_final<<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
+    double mom[4];
+    HU_CHECK(hipMemcpyAsync(mom, m->mom.p, sizeof(mom), hipMemcpyDeviceToHost, m->s));
+    HU_CHECK(hipStreamSynchronize(m->s));
+    const double gamma = sqrt(mom[1] * mom[1] + mom[3] * mom[3] + 1);
+    const double hr0 = m->geo.h[0] * gamma, hr1 = m->geo.h[1];
+    DBuf<uint8_t> cls;
+    DBuf<double> rho, b, phi, guess;
+    cls.alloc(nn); rho.alloc(nn); b.alloc(nn); phi.alloc(nn); guess.alloc(nn);
+    const mw::Fields<T> f = m->fields<T>();
+    const dim3 gg((W + mw::kB - 1) / mw::kB, H);
+    const long long maxX = (long long)(size_t)(m->p.lenBefore / m->geo.h[0]);   // Bend.hh:288-291
+    mw::k_pois_classes<T><<<gg, mw::kB, 0, m->s>>>(f, m->ws.p, maxX, cls.p);
+    HU_LAUNCH_CHECK();
+    pmg::Solver mg;
+    mg.setup(W, H, hr0, hr1, cls.p, m->s);
+    const double cc = 1 / (pm::kEps0 * gamma);
+    const int nb = (int)((nn + mw::kB - 1) / mw::kB);
+    auto rhs = [&](double drift, double shiftx) {
+        HU_CHECK(hipMemsetAsync(rho.p, 0, nn * sizeof(double), m->s));
+        if (m->p.order == 0) mw_scatter<T, 0>(m, drift, shiftx, rho.p);
+        else if (m->p.order == 1) mw_scatter<T, 1>(m, drift, shiftx, rho.p);
+        else mw_scatter<T, 2>(m, drift, shiftx, rho.p);
+        mw::k_pois_rhs<<<nb, mw::kB, 0, m->s>>>(rho.p, b.p, nn, m->geo.h[0] * m->geo.h[1], cc);
+        HU_LAUNCH_CHECK();
+    };
+    auto solve = [&](int k) {
+        const pmg::Stats st = mg.solve(b.p, phi.p, tol, maxit);
+        if (cycles) cycles[k] = st.cycles;
+        if (resid) resid[k] = st.bnorm > 0 ? st.resid / st.bnorm : st.resid;
+    };
+    // solve 1: Ey from the charge at t + dt/2
+    rhs(m->dt / 2 * pm::kC, 0.0);
+    HU_CHECK(hipMemsetAsync(phi.p, 0, nn * sizeof(double), m->s));
+    solve(0);
+    mw::k_pois_fill<T><<<gg, mw::kB, 0, m->s>>>(f, m->ws.p, phi.p, 1, gamma, hr1);
+    // solve 2: Hz with the bunch half a cell back
+    rhs(0.0, -0.5 * hr0 / gamma);
+    mw::k_pois_shift_guess<<<nb, mw::kB, 0, m->s>>>(phi.p, guess.p, W, H, 1.0);
+    std::swap(phi.p, guess.p);
+    solve(1);
+    mw::k_pois_fill<T><<<gg, mw::kB, 0, m->s>>>(f, m->ws.p, phi.p, 2, sqrt(gamma * gamma - 1),
+                                                 2 * pm::kC * pm::kMu0 * hr1);
+    // solve 3: Ex
+    rhs(0.0, 0.0);
+    mw::k_pois_shift_guess<<<nb, mw::kB, 0, m->s>>>(phi.p, guess.p, W, H, -0.5);
+    std::swap(phi.p, guess.p);
+    solve(2);
+    mw::k_pois_fill<T><<<gg, mw::kB, 0, m->s>>>(f, m->ws.p, phi.p, 0, 1.0, hr0);
+    mw::k_pois_reset<T><<<gg, mw::kB, 0, m->s>>>(f, m->ws.p);
+    HU_LAUNCH_CHECK();
+    HU_CHECK(hipStreamSynchronize(m->s));
+}
+
 extern "C" {
 
 int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
@@ -1216,6 +1431,20 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
             if (f32) sort_particles<float>(m); else sort_particles<double>(m);
         }
         HU_CHECK(hipStreamSynchronize(m->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid)
+{
+    try {
+        if (!m || !m->ready) throw Error(CSRST_ESTATE, "csrst_mw_poisson before csrst_mw_init_bunch");
+        if (m->steps != 0) throw Error(CSRST_ESTATE, "the Poisson start must precede the first step");
+        if (maxit < 0 || !(tol >= 0)) throw Error(CSRST_EINVAL, "bad tolerance or iteration limit");
+        if (m->f32) mw_poisson<float>(m, tol, maxit, cycles, resid);
+        else mw_poisson<double>(m, tol, maxit, cycles, resid);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return mw_fail(m, e);

@@ -203,6 +203,42 @@ __global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
     }
 }
 
+// The same y-line solve for columns longer than the LDS holds: one thread per
+// column, Thomas elimination (the forward coefficients go to the level's
+// residual array, free during smoothing).
+__global__ void k_line_thomas(Level L, int color)
+{
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + color;
+    if (i >= L.nx || !L.colact[i]) return;
+    const int ny = L.ny;
+    const double diag = 2.0 * (L.cx + L.cy);
+    double cp = 0.0, dp = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        const size_t k = (size_t)j * L.nx + i;
+        double a = 0.0, c = 0.0, bd = 1.0, d = 0.0;
+        if (L.act[k]) {
+            double s = 0.0;
+            if (i > 0 && L.act[k - 1]) s += L.phi[k - 1];
+            if (i + 1 < L.nx && L.act[k + 1]) s += L.phi[k + 1];
+            d = L.rhs[k] + L.cx * s;
+            a = (j > 0 && L.act[k - L.nx]) ? -L.cy : 0.0;
+            c = (j + 1 < ny && L.act[k + L.nx]) ? -L.cy : 0.0;
+            bd = diag;
+        }
+        const double m = bd - a * cp;
+        cp = c / m;
+        dp = (d - a * dp) / m;
+        L.res[k] = cp;
+        L.phi[k] = dp;
+    }
+    double x = 0.0;
+    for (int j = ny - 1; j >= 0; --j) {
+        const size_t k = (size_t)j * L.nx + i;
+        x = L.phi[k] - L.res[k] * x;
+        L.phi[k] = L.act[k] ? x : 0.0;
+    }
+}
+
 // Full weighting in x onto the coarse rhs; coarse iterate cleared.
 __global__ void k_restrict(Level f, Level c)
 {
@@ -251,7 +287,6 @@ void launch_pcr(const Level &L, int color, hipStream_t s)
 void Solver::setup(int nx, int ny, double hx, double hy, const uint8_t *cls, hipStream_t s)
 {
     if (nx < 1 || ny < 1) throw Error(CSRST_EINVAL, "empty Poisson grid");
-    if (ny > kMaxNy) throw Error(CSRST_EINVAL, hu::fmt("Poisson grid has %d rows; the line solver supports %d", ny, kMaxNy));
     if (!(hx > 0) || !(hy > 0)) throw Error(CSRST_EINVAL, "Poisson spacings must be positive");
     nx_ = nx; ny_ = ny; s_ = s; cls_ = cls;
     lv_.clear();
@@ -297,7 +332,11 @@ void Solver::smooth(int l, int sweeps)
             if (L.ny <= 256) launch_pcr<1>(L, color, s_);
             else if (L.ny <= 512) launch_pcr<2>(L, color, s_);
             else if (L.ny <= 1024) launch_pcr<4>(L, color, s_);
-            else launch_pcr<8>(L, color, s_);
+            else if (L.ny <= kMaxNy) launch_pcr<8>(L, color, s_);
+            else {
+                const int ncol = (L.nx + 1 - color) / 2;
+                if (ncol > 0) k_line_thomas<<<(ncol + 63) / 64, 64, 0, s_>>>(L, color);
+            }
         }
     HU_LAUNCH_CHECK();
 }

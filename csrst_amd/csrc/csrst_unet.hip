@@ -1,0 +1,767 @@
+// Learned warm start for the Poisson multigrid: a 4-level convolutional
+// U-Net (32 base channels) mapping a charge density to a potential, with
+// BF16 weights and activations and FP32 accumulation on the MFMA units.
+//
+// No counterpart in the reference (its Poisson start is Belos CG + ML AMG,
+// PoissonSolver.cpp:85-201); the network only supplies the initial guess and
+// the multigrid still iterates to its tolerance.
+//
+// Layout: activations NHWC (channels fastest), weights [Cout][3][3][Cin].
+// Every 3x3 convolution (stride 1, zero padding 1) is an implicit GEMM on
+// v_mfma_f32_16x16x16bf16_1k: a wavefront computes a 16 (output channel) x 64
+// (pixel) tile as four 16x16 accumulators; K runs over (tap, input channel).
+// The same kernel, with transposed weights and mirrored taps, gives the
+// input gradient; the weight gradient is a GEMM over pixels split across
+// workgroups with FP32 atomics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "csrst_hip.h"
+#include "hip_util.h"
+
+using hu::DBuf;
+using hu::Error;
+
+namespace unet {
+
+typedef short sh4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
+// round to nearest even
+__host__ __device__ __forceinline__ bf16 f2bf(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+// MFMA 16x16x16 (BF16 in, FP32 accumulate).  Lane l holds A[l%16][4(l/16)..+3],
+// B[4(l/16)..+3][l%16] and D[4(l/16)..+3][l%16].
+__device__ __forceinline__ f4 mfma(sh4 a, sh4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+__device__ __forceinline__ sh4 load4(const bf16 *p)
+{
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    sh4 r;
+    r[0] = (short)(v.x & 0xFFFF); r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xFFFF); r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+struct ConvArgs {
+    const bf16 *x;       // [P][Cin]
+    const bf16 *w;       // [CoutP][9][Cin] (rows past Cout are zero)
+    const float *bias;   // [Cout] or null
+    bf16 *y;             // [P][Cout] (bf16 out) or null
+    float *yf;           // [P][Cout] (fp32 out) or null
+    int B, H, W, Cin, Cout, CoutP;
+    int flip;            // 1: taps mirrored (input gradient)
+    int relu;
+};
+
+// Implicit-GEMM 3x3 convolution.  grid = (ceil(P / 256), CoutP / 16);
+// each of the 4 waves covers 64 consecutive pixels of the block's 256.
+__global__ void __launch_bounds__(kB) k_conv(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = a.B * a.H * a.W;
+    const int co0 = blockIdx.y * 16;
+    const int pw = blockIdx.x * 256 + wave * 64;
+    if (pw >= P) return;
+    const int li = lane & 15, kg = lane >> 4;
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    // pixel coordinates of this lane's B columns
+    int pb[4], py[4], px[4];
+    bool pv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int p = pw + 16 * t + li;
+        pv[t] = p < P;
+        const int pp = pv[t] ? p : 0;
+        px[t] = pp % a.W;
+        py[t] = (pp / a.W) % a.H;
+        pb[t] = pp / (a.W * a.H);
+    }
+    const int Cin = a.Cin;
+    if ((Cin & 15) == 0) {
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = (a.flip ? 1 - tap / 3 : tap / 3 - 1), dx = (a.flip ? 1 - tap % 3 : tap % 3 - 1);
+            int off[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int ys = py[t] + dy, xs = px[t] + dx;
+                const bool ok = pv[t] && ys >= 0 && ys < a.H && xs >= 0 && xs < a.W;
+                off[t] = ok ? ((pb[t] * a.H + ys) * a.W + xs) * Cin : -1;
+            }
+            const bf16 *wr = a.w + ((size_t)(co0 + li) * 9 + tap) * Cin;
+            for (int c0 = 0; c0 < Cin; c0 += 16) {
+                const sh4 av = load4(wr + c0 + 4 * kg);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    sh4 bv = sh4{0, 0, 0, 0};
+                    if (off[t] >= 0) bv = load4(a.x + off[t] + c0 + 4 * kg);
+                    acc[t] = mfma(av, bv, acc[t]);
+                }
+            }
+        }
+    } else {
+        // small Cin: K = 9 Cin flattened as k = tap * Cin + ci, padded to 16
+        const int K = 9 * Cin;
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            sh4 av;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = k0 + 4 * kg + r;
+                av[r] = k < K ? (short)a.w[(size_t)(co0 + li) * 9 * Cin + k] : (short)0;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sh4 bv;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = k0 + 4 * kg + r;
+                    short v = 0;
+                    if (k < K && pv[t]) {
+                        const int tap = k / Cin, ci = k % Cin;
+                        const int dy = (a.flip ? 1 - tap / 3 : tap / 3 - 1), dx = (a.flip ? 1 - tap % 3 : tap % 3 - 1);
+                        const int ys = py[t] + dy, xs = px[t] + dx;
+                        if (ys >= 0 && ys < a.H && xs >= 0 && xs < a.W)
+                            v = (short)a.x[((size_t)(pb[t] * a.H + ys) * a.W + xs) * Cin + ci];
+                    }
+                    bv[r] = v;
+                }
+                acc[t] = mfma(av, bv, acc[t]);
+            }
+        }
+    }
+    // epilogue: rows co0 + 4 kg + r, column = pixel
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (!pv[t]) continue;
+        const size_t p = (size_t)pw + 16 * t + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = co0 + 4 * kg + r;
+            if (co >= a.Cout) continue;
+            float v = acc[t][r] + (a.bias ? a.bias[co] : 0.f);
+            if (a.relu) v = v > 0.f ? v : 0.f;
+            if (a.y) a.y[p * a.Cout + co] = f2bf(v);
+            if (a.yf) a.yf[p * a.Cout + co] = v;
+        }
+    }
+}
+
+// Weight gradient dW[co][tap][ci] += sum_p dY[p][co] X[p + s(tap)][ci]:
+// M = Cout, N = 9 Cin (n = tap * Cin + ci), K = pixels, split over blockIdx.z.
+// grid = (ceil(9 Cin / 64), CoutP / 16, splits); 64 threads per block.
+__global__ void __launch_bounds__(64) k_conv_wgrad(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
+                                                  int Cin, int Cout, int chunk)
+{
+    const int lane = threadIdx.x, li = lane & 15, kg = lane >> 4;
+    const int P = B * H * W, N = 9 * Cin;
+    const int co0 = blockIdx.y * 16;
+    const int n0 = blockIdx.x * 64;
+    const int k0 = blockIdx.z * chunk, k1 = min(P, k0 + chunk);
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    int tapn[4], cin[4];
+    bool nv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int n = n0 + 16 * t + li;
+        nv[t] = n < N;
+        tapn[t] = nv[t] ? n / Cin : 0;
+        cin[t] = nv[t] ? n % Cin : 0;
+    }
+    const int co = co0 + li;
+    for (int kb = k0; kb < k1; kb += 16) {
+        sh4 av;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int p = kb + 4 * kg + r;
+            av[r] = (p < k1 && co < Cout) ? (short)dy[(size_t)p * Cout + co] : (short)0;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            sh4 bv;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = kb + 4 * kg + r;
+                short v = 0;
+                if (p < k1 && nv[t]) {
+                    const int xx = p % W, yy = (p / W) % H, bb = p / (W * H);
+                    const int ys = yy + tapn[t] / 3 - 1, xs = xx + tapn[t] % 3 - 1;
+                    if (ys >= 0 && ys < H && xs >= 0 && xs < W)
+                        v = (short)x[((size_t)(bb * H + ys) * W + xs) * Cin + cin[t]];
+                }
+                bv[r] = v;
+            }
+            acc[t] = mfma(av, bv, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (!nv[t]) continue;
+        const int n = n0 + 16 * t + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = co0 + 4 * kg + r;
+            if (c < Cout) atomicAdd(&dw[(size_t)c * N + n], acc[t][r]);
+        }
+    }
+}
+
+// bias gradient: db[co] += sum_p dY[p][co]
+__global__ void k_bias_grad(const bf16 *dy, float *db, int P, int C)
+{
+    __shared__ float sh[kB];
+    const int c = blockIdx.y;
+    float s = 0.f;
+    for (int p = blockIdx.x * kB + threadIdx.x; p < P; p += gridDim.x * kB) s += bf2f(dy[(size_t)p * C + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&db[c], sh[0]);
+}
+
+// dPre = dY * (Y > 0)
+__global__ void k_relu_grad(const bf16 *dy, const bf16 *y, bf16 *out, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) out[k] = bf2f(y[k]) > 0.f ? dy[k] : (bf16)0;
+}
+
+// 2x2 average pool (NHWC); H, W even
+__global__ void k_pool(const bf16 *x, bf16 *y, int B, int H, int W, int C)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    if (k >= (size_t)B * Ho * Wo * C) return;
+    const int c = (int)(k % C);
+    size_t r = k / C;
+    const int xo = (int)(r % Wo); r /= Wo;
+    const int yo = (int)(r % Ho);
+    const int b = (int)(r / Ho);
+    auto at = [&](int yy, int xx) { return bf2f(x[(((size_t)b * H + yy) * W + xx) * C + c]); };
+    y[k] = f2bf(0.25f * (at(2 * yo, 2 * xo) + at(2 * yo, 2 * xo + 1) + at(2 * yo + 1, 2 * xo) + at(2 * yo + 1, 2 * xo + 1)));
+}
+
+__global__ void k_pool_grad(const bf16 *dy, bf16 *dx, int B, int H, int W, int C)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k >= (size_t)B * H * W * C) return;
+    const int c = (int)(k % C);
+    size_t r = k / C;
+    const int xx = (int)(r % W); r /= W;
+    const int yy = (int)(r % H);
+    const int b = (int)(r / H);
+    dx[k] = f2bf(0.25f * bf2f(dy[(((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]));
+}
+
+// nearest x2 upsample plus skip: y = up(x) + s
+__global__ void k_up_add(const bf16 *x, const bf16 *s, bf16 *y, int B, int H, int W, int C)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k >= (size_t)B * H * W * C) return;
+    const int c = (int)(k % C);
+    size_t r = k / C;
+    const int xx = (int)(r % W); r /= W;
+    const int yy = (int)(r % H);
+    const int b = (int)(r / H);
+    y[k] = f2bf(bf2f(x[(((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]) + bf2f(s[k]));
+}
+
+// gradient of up(x): dx[y][x] = sum of the 2x2 block of dy
+__global__ void k_up_grad(const bf16 *dy, bf16 *dx, int B, int Hs, int Ws, int C)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k >= (size_t)B * Hs * Ws * C) return;
+    const int c = (int)(k % C);
+    size_t r = k / C;
+    const int xx = (int)(r % Ws); r /= Ws;
+    const int yy = (int)(r % Hs);
+    const int b = (int)(r / Hs);
+    const int H = 2 * Hs, W = 2 * Ws;
+    auto at = [&](int y2, int x2) { return bf2f(dy[(((size_t)b * H + y2) * W + x2) * C + c]); };
+    dx[k] = f2bf(at(2 * yy, 2 * xx) + at(2 * yy, 2 * xx + 1) + at(2 * yy + 1, 2 * xx) + at(2 * yy + 1, 2 * xx + 1));
+}
+
+__global__ void k_add_bf(const bf16 *a, const bf16 *b, bf16 *y, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) y[k] = f2bf(bf2f(a[k]) + bf2f(b[k]));
+}
+
+__global__ void k_to_bf(const float *x, bf16 *y, size_t n, float scale)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) y[k] = f2bf(x[k] * scale);
+}
+
+// MSE loss of the fp32 output against the target; gradient 2 (o - t) / n in bf16
+__global__ void k_mse(const float *o, const float *t, bf16 *g, float *loss, size_t n)
+{
+    __shared__ float sh[kB];
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    float d = 0.f;
+    if (k < n) {
+        d = o[k] - t[k];
+        g[k] = f2bf(2.f * d / (float)n);
+    }
+    sh[threadIdx.x] = d * d;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, sh[0] / (float)n);
+}
+
+// Adam on the fp32 master weights; refresh the bf16 copies (padded rows stay 0)
+__global__ void k_adam(float *w, float *m, float *v, const float *g, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k >= n) return;
+    const float gg = g[k];
+    const float mm = b1 * m[k] + (1.f - b1) * gg;
+    const float vv = b2 * v[k] + (1.f - b2) * gg * gg;
+    m[k] = mm;
+    v[k] = vv;
+    w[k] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// bf16 copies of one layer: wf [Cout][9][Cin] -> wb [CoutP][9][Cin] and the
+// transposed wt [CinP][9][Cout] (wt[ci][tap][co] = w[co][tap][ci]) used by the
+// input gradient; padded rows are zero.
+__global__ void k_wcopy(const float *wf, bf16 *wb, bf16 *wt, int Cin, int Cout, int CoutP, int CinP)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    const size_t nb = (size_t)CoutP * 9 * Cin, nt = (size_t)CinP * 9 * Cout;
+    if (k < nb) {
+        const int co = (int)(k / (9 * Cin));
+        wb[k] = co < Cout ? f2bf(wf[k]) : (bf16)0;
+    }
+    if (k < nt) {
+        const int ci = (int)(k / (9 * Cout)), rem = (int)(k % (9 * Cout)), tap = rem / Cout, co = rem % Cout;
+        wt[k] = ci < Cin ? f2bf(wf[((size_t)co * 9 + tap) * Cin + ci]) : (bf16)0;
+    }
+}
+
+int nb(size_t n) { return (int)((n + kB - 1) / kB); }
+int pad16(int c) { return (c + 15) / 16 * 16; }
+
+struct Layer {
+    int cin, cout, relu;
+    size_t woff, boff;       // offsets into the fp32 parameter vector
+};
+
+}  // namespace unet
+
+using namespace unet;
+
+struct csrst_warmstart {
+    std::string err;
+    hipStream_t s = nullptr;
+    int H = 0, W = 0, base = 32, maxBatch = 0;
+    std::vector<Layer> L;          // 15 convolutions
+    size_t nparam = 0;
+    DBuf<float> w, m, v, g;        // master weights, Adam moments, gradients
+    DBuf<bf16> wb, wt;             // bf16 copies per layer (offsets below)
+    std::vector<size_t> wbOff, wtOff;
+    DBuf<bf16> act;                // activations and gradients arena
+    DBuf<float> outf, tgt, scratchF, lossBuf;
+    int steps = 0;
+    // activation offsets
+    struct Buf { size_t off; int h, w, c; };
+    std::vector<Buf> A;            // forward activations
+    std::vector<Buf> Gd;           // gradient buffers (same shapes)
+    int batch = 0;
+
+    bf16 *ptr(const Buf &b) { return act.p + b.off; }
+};
+
+namespace {
+
+// conv indices in order: 0 c00 1->32, 1 c01, 2 c10 32->64, 3 c11, 4 c20 64->128,
+// 5 c21, 6 c30 128->256, 7 c31, 8 c22 256->128, 9 c23, 10 c12 128->64, 11 c13,
+// 12 c02 64->32, 13 c03, 14 out 32->1
+void make_layers(csrst_warmstart *u)
+{
+    const int b = u->base;
+    const int io[15][3] = {{1, b, 1},          {b, b, 1},          {b, 2 * b, 1},     {2 * b, 2 * b, 1},
+                           {2 * b, 4 * b, 1},  {4 * b, 4 * b, 1},  {4 * b, 8 * b, 1}, {8 * b, 8 * b, 1},
+                           {8 * b, 4 * b, 1},  {4 * b, 4 * b, 1},  {4 * b, 2 * b, 1}, {2 * b, 2 * b, 1},
+                           {2 * b, b, 1},      {b, b, 1},          {b, 1, 0}};
+    size_t off = 0;
+    u->L.clear();
+    for (auto &e : io) {
+        Layer l;
+        l.cin = e[0]; l.cout = e[1]; l.relu = e[2];
+        l.woff = off; off += (size_t)l.cout * 9 * l.cin;
+        l.boff = off; off += l.cout;
+        u->L.push_back(l);
+    }
+    u->nparam = off;
+}
+
+void conv(csrst_warmstart *u, int li, const bf16 *x, bf16 *y, float *yf, int B, int H, int W)
+{
+    const Layer &l = u->L[li];
+    ConvArgs a;
+    a.x = x; a.w = u->wb.p + u->wbOff[li]; a.bias = u->w.p + l.boff; a.y = y; a.yf = yf;
+    a.B = B; a.H = H; a.W = W; a.Cin = l.cin; a.Cout = l.cout; a.CoutP = pad16(l.cout); a.flip = 0; a.relu = l.relu;
+    const int P = B * H * W;
+    k_conv<<<dim3((P + 255) / 256, a.CoutP / 16), kB, 0, u->s>>>(a);
+}
+
+// input gradient of conv li: dx[P][cin] from dpre[P][cout]
+void conv_dgrad(csrst_warmstart *u, int li, const bf16 *dpre, bf16 *dx, int B, int H, int W)
+{
+    const Layer &l = u->L[li];
+    ConvArgs a;
+    a.x = dpre; a.w = u->wt.p + u->wtOff[li]; a.bias = nullptr; a.y = dx; a.yf = nullptr;
+    a.B = B; a.H = H; a.W = W; a.Cin = l.cout; a.Cout = l.cin; a.CoutP = pad16(l.cin); a.flip = 1; a.relu = 0;
+    const int P = B * H * W;
+    k_conv<<<dim3((P + 255) / 256, a.CoutP / 16), kB, 0, u->s>>>(a);
+}
+
+void conv_wgrad(csrst_warmstart *u, int li, const bf16 *dpre, const bf16 *x, int B, int H, int W)
+{
+    const Layer &l = u->L[li];
+    const int P = B * H * W;
+    const int chunk = 1024;
+    dim3 grid((9 * l.cin + 63) / 64, pad16(l.cout) / 16, (P + chunk - 1) / chunk);
+    k_conv_wgrad<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
+    dim3 bg(std::min(64, (P + kB - 1) / kB), l.cout);
+    k_bias_grad<<<bg, kB, 0, u->s>>>(dpre, u->g.p + l.boff, P, l.cout);
+}
+
+void refresh_bf16(csrst_warmstart *u)
+{
+    for (size_t i = 0; i < u->L.size(); ++i) {
+        const Layer &l = u->L[i];
+        const size_t n = std::max((size_t)pad16(l.cout) * 9 * l.cin, (size_t)pad16(l.cin) * 9 * l.cout);
+        k_wcopy<<<nb(n), kB, 0, u->s>>>(u->w.p + l.woff, u->wb.p + u->wbOff[i], u->wt.p + u->wtOff[i], l.cin, l.cout,
+                                         pad16(l.cout), pad16(l.cin));
+    }
+    HU_LAUNCH_CHECK();
+}
+
+// activation plan for batch B: forward buffers A[0..] and gradient buffers
+void plan(csrst_warmstart *u, int B)
+{
+    if (u->batch == B) return;
+    const int H = u->H, W = u->W, b = u->base;
+    u->A.clear();
+    size_t off = 0;
+    auto add = [&](std::vector<csrst_warmstart::Buf> &v, int h, int w, int c) {
+        v.push_back({off, h, w, c});
+        off += (size_t)B * h * w * c;
+        off = (off + 63) / 64 * 64;
+    };
+    // forward: 0 in, 1 c00, 2 s0(c01), 3 p0, 4 c10, 5 s1(c11), 6 p1, 7 c20, 8 s2(c21), 9 p2, 10 c30, 11 c31,
+    // 12 c22, 13 u2(up+s2), 14 c23, 15 c12, 16 u1, 17 c13, 18 c02, 19 u0, 20 c03, 21 out(bf16)
+    const int sh[22][3] = {{H, W, 1},          {H, W, b},          {H, W, b},          {H / 2, W / 2, b},
+                           {H / 2, W / 2, 2 * b}, {H / 2, W / 2, 2 * b}, {H / 4, W / 4, 2 * b}, {H / 4, W / 4, 4 * b},
+                           {H / 4, W / 4, 4 * b}, {H / 8, W / 8, 4 * b}, {H / 8, W / 8, 8 * b}, {H / 8, W / 8, 8 * b},
+                           {H / 8, W / 8, 4 * b}, {H / 4, W / 4, 4 * b}, {H / 4, W / 4, 4 * b}, {H / 4, W / 4, 2 * b},
+                           {H / 2, W / 2, 2 * b}, {H / 2, W / 2, 2 * b}, {H / 2, W / 2, b},     {H, W, b},
+                           {H, W, b},          {H, W, 1}};
+    for (auto &e : sh) add(u->A, e[0], e[1], e[2]);
+    u->Gd.clear();
+    for (auto &e : sh) add(u->Gd, e[0], e[1], e[2]);
+    // two scratch gradient buffers of the largest activation
+    size_t big = 0;
+    for (auto &e : sh) big = std::max(big, (size_t)B * e[0] * e[1] * e[2]);
+    u->Gd.push_back({off, 0, 0, 0}); off += (big + 63) / 64 * 64;
+    u->Gd.push_back({off, 0, 0, 0}); off += (big + 63) / 64 * 64;
+    u->act.alloc(off);
+    u->outf.alloc((size_t)B * H * W);
+    u->tgt.alloc((size_t)B * H * W);
+    u->batch = B;
+}
+
+void forward(csrst_warmstart *u, int B)
+{
+    auto &A = u->A;
+    const int H = u->H, W = u->W;
+    auto P = [&](int i) { return u->ptr(A[i]); };
+    auto pool = [&](int src, int dst) {
+        const auto &d = A[dst];
+        k_pool<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(P(src), P(dst), B, A[src].h, A[src].w, d.c);
+    };
+    auto upadd = [&](int src, int skip, int dst) {
+        const auto &d = A[dst];
+        k_up_add<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(P(src), P(skip), P(dst), B, d.h, d.w, d.c);
+    };
+    conv(u, 0, P(0), P(1), nullptr, B, H, W);
+    conv(u, 1, P(1), P(2), nullptr, B, H, W);
+    pool(2, 3);
+    conv(u, 2, P(3), P(4), nullptr, B, H / 2, W / 2);
+    conv(u, 3, P(4), P(5), nullptr, B, H / 2, W / 2);
+    pool(5, 6);
+    conv(u, 4, P(6), P(7), nullptr, B, H / 4, W / 4);
+    conv(u, 5, P(7), P(8), nullptr, B, H / 4, W / 4);
+    pool(8, 9);
+    conv(u, 6, P(9), P(10), nullptr, B, H / 8, W / 8);
+    conv(u, 7, P(10), P(11), nullptr, B, H / 8, W / 8);
+    conv(u, 8, P(11), P(12), nullptr, B, H / 8, W / 8);
+    upadd(12, 8, 13);
+    conv(u, 9, P(13), P(14), nullptr, B, H / 4, W / 4);
+    conv(u, 10, P(14), P(15), nullptr, B, H / 4, W / 4);
+    upadd(15, 5, 16);
+    conv(u, 11, P(16), P(17), nullptr, B, H / 2, W / 2);
+    conv(u, 12, P(17), P(18), nullptr, B, H / 2, W / 2);
+    upadd(18, 2, 19);
+    conv(u, 13, P(19), P(20), nullptr, B, H, W);
+    conv(u, 14, P(20), P(21), u->outf.p, B, H, W);
+    HU_LAUNCH_CHECK();
+}
+
+// backward from the bf16 output gradient in Gd[21]; fills u->g
+void backward(csrst_warmstart *u, int B)
+{
+    auto &A = u->A;
+    auto &G = u->Gd;
+    const int H = u->H, W = u->W;
+    auto P = [&](int i) { return u->ptr(A[i]); };
+    auto D = [&](int i) { return u->ptr(G[i]); };
+    bf16 *pre = u->ptr(G[22]);
+    HU_CHECK(hipMemsetAsync(u->g.p, 0, u->nparam * sizeof(float), u->s));
+    // conv li with input activation xin and output activation yout; dy given in D(yout)
+    auto back = [&](int li, int xin, int yout, int h, int w, bool needDx) {
+        const Layer &l = u->L[li];
+        const size_t n = (size_t)B * h * w * l.cout;
+        const bf16 *dpre = D(yout);
+        if (l.relu) {
+            k_relu_grad<<<nb(n), kB, 0, u->s>>>(D(yout), P(yout), pre, n);
+            dpre = pre;
+        }
+        conv_wgrad(u, li, dpre, P(xin), B, h, w);
+        if (needDx) conv_dgrad(u, li, dpre, D(xin), B, h, w);
+    };
+    auto addgrad = [&](int dst, const bf16 *src) {   // D(dst) += src
+        const auto &d = A[dst];
+        const size_t n = (size_t)B * d.h * d.w * d.c;
+        k_add_bf<<<nb(n), kB, 0, u->s>>>(D(dst), src, D(dst), n);
+    };
+    auto upgrad = [&](int dstLow, int srcHigh) {      // D(dstLow) = up^T D(srcHigh)
+        const auto &d = A[dstLow];
+        k_up_grad<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(D(srcHigh), D(dstLow), B, d.h, d.w, d.c);
+    };
+    auto poolgrad = [&](bf16 *dst, int srcLow, int hiIdx) {   // dst = pool^T D(srcLow)
+        const auto &d = A[hiIdx];
+        k_pool_grad<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(D(srcLow), dst, B, d.h, d.w, d.c);
+    };
+    bf16 *tmp = u->ptr(G[23]);
+    back(14, 20, 21, H, W, true);      // out -> D20
+    back(13, 19, 20, H, W, true);      // -> D19 (= d(up(c02)) and d(s0))
+    upgrad(18, 19);                    // D18 = up^T D19
+    back(12, 17, 18, H / 2, W / 2, true);
+    back(11, 16, 17, H / 2, W / 2, true);
+    upgrad(15, 16);
+    back(10, 14, 15, H / 4, W / 4, true);
+    back(9, 13, 14, H / 4, W / 4, true);
+    upgrad(12, 13);
+    back(8, 11, 12, H / 8, W / 8, true);
+    back(7, 10, 11, H / 8, W / 8, true);
+    back(6, 9, 10, H / 8, W / 8, true);
+    // D8 = pool^T D9 + D13 (skip)
+    poolgrad(D(8), 9, 8);
+    addgrad(8, D(13));
+    back(5, 7, 8, H / 4, W / 4, true);
+    back(4, 6, 7, H / 4, W / 4, true);
+    poolgrad(D(5), 6, 5);
+    addgrad(5, D(16));
+    back(3, 4, 5, H / 2, W / 2, true);
+    back(2, 3, 4, H / 2, W / 2, true);
+    poolgrad(D(2), 3, 2);
+    addgrad(2, D(19));
+    back(1, 1, 2, H, W, true);
+    back(0, 0, 1, H, W, false);
+    (void)tmp;
+    HU_LAUNCH_CHECK();
+}
+
+void upload_input(csrst_warmstart *u, const float *rho, int B, float scale)
+{
+    const size_t n = (size_t)B * u->H * u->W;
+    u->scratchF.alloc(std::max(u->scratchF.n, n));
+    HU_CHECK(hipMemcpyAsync(u->scratchF.p, rho, n * sizeof(float), hipMemcpyHostToDevice, u->s));
+    k_to_bf<<<nb(n), kB, 0, u->s>>>(u->scratchF.p, u->ptr(u->A[0]), n, scale);
+    HU_LAUNCH_CHECK();
+}
+
+int ws_fail(csrst_warmstart *u, const std::exception &e)
+{
+    const Error *he = dynamic_cast<const Error *>(&e);
+    if (u) u->err = e.what();
+    hu::global_error() = e.what();
+    return he ? he->code : CSRST_EINVAL;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csrst_warmstart_create(int H, int W, int maxBatch, csrst_warmstart **out)
+{
+    std::unique_ptr<csrst_warmstart> u(new csrst_warmstart());
+    try {
+        if (!out || H < 16 || W < 16 || H % 8 || W % 8 || maxBatch < 1)
+            throw Error(CSRST_EINVAL, "grid must be at least 16 x 16 and divisible by 8; batch >= 1");
+        u->H = H; u->W = W; u->maxBatch = maxBatch;
+        HU_CHECK(hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking));
+        make_layers(u.get());
+        u->w.alloc(u->nparam); u->m.alloc(u->nparam); u->v.alloc(u->nparam); u->g.alloc(u->nparam);
+        u->m.zero(u->s); u->v.zero(u->s); u->g.zero(u->s);
+        size_t ob = 0, ot = 0;
+        for (const Layer &l : u->L) {
+            u->wbOff.push_back(ob); ob += (size_t)pad16(l.cout) * 9 * l.cin;
+            u->wtOff.push_back(ot); ot += (size_t)pad16(l.cin) * 9 * l.cout;
+        }
+        u->wb.alloc(ob); u->wt.alloc(ot);
+        u->lossBuf.alloc(1);
+        // He initialisation from a fixed LCG (weights), zero biases
+        std::vector<float> h(u->nparam, 0.f);
+        uint64_t st = 0x9E3779B97F4A7C15ull;
+        auto uni = [&]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        for (const Layer &l : u->L) {
+            const double sd = std::sqrt(2.0 / (9.0 * l.cin));
+            for (size_t k = 0; k < (size_t)l.cout * 9 * l.cin; ++k) {
+                const double a = uni(), b = uni();
+                h[l.woff + k] = (float)(sd * std::sqrt(-2 * std::log(a)) * std::cos(6.283185307179586 * b));
+            }
+        }
+        HU_CHECK(hipMemcpyAsync(u->w.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, u->s));
+        refresh_bf16(u.get());
+        HU_CHECK(hipStreamSynchronize(u->s));
+        *out = u.release();
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        if (u && u->s) { (void)hipStreamSynchronize(u->s); (void)hipStreamDestroy(u->s); u->s = nullptr; }
+        return ws_fail(nullptr, e);
+    }
+}
+
+int csrst_warmstart_destroy(csrst_warmstart *u)
+{
+    if (!u) return CSRST_OK;
+    if (u->s) { (void)hipStreamSynchronize(u->s); (void)hipStreamDestroy(u->s); }
+    delete u;
+    return CSRST_OK;
+}
+
+int csrst_warmstart_num_params(csrst_warmstart *u, int *n)
+{
+    if (!u || !n) return CSRST_EINVAL;
+    *n = (int)u->nparam;
+    return CSRST_OK;
+}
+
+int csrst_warmstart_get_weights(csrst_warmstart *u, float *w, int n)
+{
+    try {
+        if (!u || !w || n != (int)u->nparam) throw Error(CSRST_EINVAL, "weight count mismatch");
+        HU_CHECK(hipMemcpyAsync(w, u->w.p, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->s));
+        HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+int csrst_warmstart_set_weights(csrst_warmstart *u, const float *w, int n)
+{
+    try {
+        if (!u || !w || n != (int)u->nparam) throw Error(CSRST_EINVAL, "weight count mismatch");
+        HU_CHECK(hipMemcpyAsync(u->w.p, w, u->nparam * sizeof(float), hipMemcpyHostToDevice, u->s));
+        u->m.zero(u->s); u->v.zero(u->s);
+        u->steps = 0;
+        refresh_bf16(u);
+        HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+// Inference: rho [B][H][W] (host, float) scaled by `scale`, output [B][H][W].
+int csrst_warmstart_infer(csrst_warmstart *u, const float *rho, float *out, int B, float scale)
+{
+    try {
+        if (!u || !rho || !out || B < 1 || B > u->maxBatch) throw Error(CSRST_EINVAL, "bad inference arguments");
+        plan(u, B);
+        upload_input(u, rho, B, scale);
+        forward(u, B);
+        HU_CHECK(hipMemcpyAsync(out, u->outf.p, (size_t)B * u->H * u->W * sizeof(float), hipMemcpyDeviceToHost, u->s));
+        HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+// Loss and parameter gradients of one batch (no update): grads [nparam].
+int csrst_warmstart_gradients(csrst_warmstart *u, const float *rho, const float *phi, int B, float scale, float *loss,
+                              float *grads, int n)
+{
+    try {
+        if (!u || !rho || !phi || B < 1 || B > u->maxBatch) throw Error(CSRST_EINVAL, "bad gradient arguments");
+        if (grads && n != (int)u->nparam) throw Error(CSRST_EINVAL, "gradient count mismatch");
+        plan(u, B);
+        upload_input(u, rho, B, scale);
+        const size_t np = (size_t)B * u->H * u->W;
+        HU_CHECK(hipMemcpyAsync(u->tgt.p, phi, np * sizeof(float), hipMemcpyHostToDevice, u->s));
+        forward(u, B);
+        HU_CHECK(hipMemsetAsync(u->lossBuf.p, 0, sizeof(float), u->s));
+        k_mse<<<nb(np), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
+        backward(u, B);
+        if (loss) HU_CHECK(hipMemcpyAsync(loss, u->lossBuf.p, sizeof(float), hipMemcpyDeviceToHost, u->s));
+        if (grads) HU_CHECK(hipMemcpyAsync(grads, u->g.p, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->s));
+        HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+// One Adam step on a batch; loss before the update.
+int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float *phi, int B, float scale, float lr,
+                               float *loss)
+{
+    try {
+        const int rc = csrst_warmstart_gradients(u, rho, phi, B, scale, loss, nullptr, 0);
+        if (rc) return rc;
+        ++u->steps;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)u->steps), c2 = 1.f - std::pow(b2, (float)u->steps);
+        k_adam<<<nb(u->nparam), kB, 0, u->s>>>(u->w.p, u->m.p, u->v.p, u->g.p, u->nparam, lr, b1, b2, 1e-8f, c1, c2);
+        refresh_bf16(u);
+        HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+}  // extern "C"

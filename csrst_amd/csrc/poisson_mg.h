@@ -12,7 +12,8 @@
 // (hy / (gamma hx))^2, so the smoother is an exact y-line solve (parallel
 // cyclic reduction in LDS, one workgroup per column, red/black column order)
 // and coarsening halves x only.  That pairing converges at a rate that does not
-// depend on the anisotropy.
+// depend on the anisotropy.  Lines longer than kMaxNy are solved by one
+// thread per column (Thomas elimination) instead.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,7 +27,7 @@ namespace pmg {
 
 enum NodeClass : uint8_t { kFixed = 0, kUnknown = 1, kKnown = 2 };
 
-constexpr int kMaxNy = 2048;     // y-line length the LDS solve supports
+constexpr int kMaxNy = 2048;     // longest y-line solved in LDS; longer lines use per-thread Thomas
 
 struct Level {
     int nx = 0, ny = 0;

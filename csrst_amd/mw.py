@@ -40,6 +40,14 @@ class MovingWindowStep:
         self.n = len(R)
         check(_lib.lib().csrst_mw_init_bunch(self.h, dptr(R), dptr(P), dptr(Q), C.c_int(self.n)), self.h)
 
+    def poisson(self, tol=1e-12, maxit=500):
+        """Poisson field start on the window (after init_bunch, before the
+        first step); returns the V-cycles and relative residual per solve."""
+        cyc = (C.c_int * 3)()
+        res = (C.c_double * 3)()
+        check(_lib.lib().csrst_mw_poisson(self.h, C.c_double(tol), C.c_int(maxit), cyc, res), self.h)
+        return dict(cycles=list(cyc), resid=list(res))
+
     def place(self, wx0, wy0):
         self.n = 0
         check(_lib.lib().csrst_mw_place(self.h, C.c_int(wx0), C.c_int(wy0)), self.h)

@@ -132,6 +132,15 @@ int csrst_mw_wake(csrst_mw *m, double *out, int cap, int *count);
  * [0] push+deposit, [1] field sweeps, [2] window move, [3] sort, [4] wake (ms). */
 int csrst_mw_timing(csrst_mw *m, double *ms, int cap);
 
+/* Poisson start of the moving window: after csrst_mw_init_bunch, before the
+ * first step, the three solves of PoissonSolver::computeField
+ * (PoissonSolver.cpp:85-201) on the window grid, fillFields and
+ * resetOutsideDomain.  The window edges are Dirichlet (the reference's grid
+ * is the whole chamber; its boosted field decays over width/gamma, far inside
+ * any window that holds the bunch).  Multigrid in FP64; the fields are stored
+ * in the window's precision. */
+int csrst_mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid);
+
 /* ---- Poisson initialisation ------------------------------------------------
  * Replaces PoissonSolver::computeField (PoissonSolver.cpp:85-201), whose
  * Belos block-CG + ML AMG solves the cutoffStencil system
@@ -154,6 +163,29 @@ int csrst_mw_timing(csrst_mw *m, double *ms, int cap);
 int csrst_poisson_compute_field(csrst_ctx *c, double tol, int maxit, int *cycles, double *resid);
 int csrst_poisson_solve(int nx, int ny, double hx, double hy, const uint8_t *cls, const double *b,
                         double *phi, double tol, int maxit, int *cycles, double *resid);
+
+/* ---- learned warm start for the Poisson multigrid ---------------------------
+ * No counterpart in the reference (its Poisson start iterates from a random
+ * guess, PoissonSolver.cpp:158).  A 4-level convolutional U-Net, 32 base
+ * channels, BF16 weights and activations, FP32 accumulation on MFMA, mapping
+ * a charge density on an H x W node grid to a potential that seeds
+ * csrst_poisson_solve; the multigrid still iterates to its tolerance.
+ * Arrays are [B][H][W] floats (host); `scale` multiplies the input density
+ * (the network is trained on densities of order one and the Poisson problem
+ * is linear, so callers scale phi back by 1/scale). */
+typedef struct csrst_warmstart csrst_warmstart;
+int csrst_warmstart_create(int H, int W, int maxBatch, csrst_warmstart **out);
+int csrst_warmstart_destroy(csrst_warmstart *u);
+int csrst_warmstart_num_params(csrst_warmstart *u, int *n);
+int csrst_warmstart_get_weights(csrst_warmstart *u, float *w, int n);
+int csrst_warmstart_set_weights(csrst_warmstart *u, const float *w, int n);
+int csrst_warmstart_infer(csrst_warmstart *u, const float *rho, float *out, int B, float scale);
+/* mean-square loss against phi and the FP32 gradient of every parameter */
+int csrst_warmstart_gradients(csrst_warmstart *u, const float *rho, const float *phi, int B, float scale,
+                              float *loss, float *grads, int n);
+/* one Adam step (beta 0.9 / 0.999); *loss is the batch loss before the update */
+int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float *phi, int B, float scale,
+                               float lr, float *loss);
 
 #ifdef __cplusplus
 }

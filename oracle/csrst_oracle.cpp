@@ -1041,7 +1041,8 @@ void poisson_classes(const Bend &b, std::vector<uint8_t> &flag, std::vector<uint
  * off the grid count as unflagged).  Identity rows: phi = b.  A flagged
  * neighbour that is an identity row is a known value and moves to the
  * right-hand side.  The stencil rows form an SPD system, factored by a
- * banded Cholesky in column order (y fastest, bandwidth = box height). */
+ * banded Cholesky with the nodes numbered along the shorter side of their
+ * bounding box (bandwidth = that side). */
 int poisson_direct(int nx, int ny, double hx, double hy, const uint8_t *flag, const uint8_t *active,
                    const double *b, double *phi)
 {
@@ -1058,16 +1059,20 @@ int poisson_direct(int nx, int ny, double hx, double hy, const uint8_t *flag, co
     if (i1 < 0) return 0;
     const int bw = j1 - j0 + 1, bx = i1 - i0 + 1;
     const long n = (long)bw * bx;
-    const int p = bw;                                   /* bandwidth */
+    const bool colOrder = bw <= bx;                     /* number along the shorter side */
+    const int p = colOrder ? bw : bx;                   /* bandwidth */
     std::vector<double> L((size_t)n * (p + 1), 0.0);    /* L[k][d] = L(k, k-d) */
     std::vector<double> rhs(n, 0.0);
-    auto kid = [&](int i, int j) { return (long)(i - i0) * bw + (j - j0); };
+    auto kid = [&](int i, int j) {
+        return colOrder ? (long)(i - i0) * bw + (j - j0) : (long)(j - j0) * bx + (i - i0);
+    };
     auto isflag = [&](int i, int j) { return i >= 0 && j >= 0 && i < nx && j < ny && flag[(size_t)j * nx + i]; };
     auto isact = [&](int i, int j) { return i >= 0 && j >= 0 && i < nx && j < ny && active[(size_t)j * nx + i]; };
     for (int i = i0; i <= i1; ++i)
         for (int j = j0; j <= j1; ++j) {
             const long k = kid(i, j);
             double *Lk = &L[(size_t)k * (p + 1)];
+            for (int d = 0; d <= p; ++d) Lk[d] = 0.0;
             if (!active[(size_t)j * nx + i]) { Lk[0] = 1.0; rhs[k] = 0.0; continue; }
             Lk[0] = -2 * (EW + NS);
             rhs[k] = b[(size_t)j * nx + i];

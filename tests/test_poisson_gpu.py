@@ -56,7 +56,18 @@ def test_multigrid_rejects_bad_input(gpu):
     with pytest.raises(Exception):
         P.solve(1.0, 1.0, cls, np.zeros((4, 4)))
     with pytest.raises(Exception):
-        P.solve(1.0, 1.0, np.ones((4097, 2), np.uint8), np.zeros((4097, 2)))
+        P.solve(1.0, 1.0, np.ones((4, 4), np.uint8), np.zeros((4, 5)))
+
+
+def test_multigrid_long_lines(gpu):
+    """Columns longer than the LDS line solver takes use the per-thread path."""
+    flag, active, b = masked_grid(24, 2600, 4)
+    cls = P.classes(flag, active)
+    phi, cyc, res = P.solve(50 * 2e-4, 2e-4, cls, b, tol=1e-12, maxit=60)
+    print("2600 rows: %d cycles, residual %.2e" % (cyc, res))
+    assert res <= 1e-12
+    ref = O.poisson_solve(50 * 2e-4, 2e-4, flag, active, b)
+    assert np.abs(phi - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("order", [0, 1, 2])
@@ -97,4 +108,41 @@ def test_compute_field_matches_oracle(gpu, order):
     assert g.info()["pos"] == o.info()["pos"]
     for w in ("mE", "mH"):
         bad += close(g.field(w), o.field(w), 1e-8, w + " after 4 steps")
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("fp32,tol", [(False, 1e-9), (True, 2e-6)])
+def test_moving_window_poisson_matches_oracle(gpu, fp32, tol):
+    """The window's Poisson start against the oracle's global three-solve
+    start: same operator, rho and fillFields; the window edges (Dirichlet
+    here) lie outside the chamber or >= 30 cells ahead of the bunch, where the
+    boosted field has decayed to nothing."""
+    from csrst_amd.mw import MovingWindowStep
+    Nx, Ny, n = 1100, 120, 3000
+    b = O.Bend(Nx=Nx, Ny=Ny, **GEOM)
+    hx, hy = b.hr
+    R, Pm, Q = gaussian_bunch(n, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=5)
+    m = MovingWindowStep(600, 64, hx, hy, order=2, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
+    m.init_bunch(R, Pm, Q)
+    st = m.poisson(tol=1e-13, maxit=60)
+    print("window Poisson", st)
+    assert all(r <= 1e-13 for r in st["resid"])
+    wx0, wy0 = m.info()["pos"]
+    F = m.fields()
+    o = O.Sim(Nx, Ny, order=2, statModulus=1, **GEOM)
+    o.init(R, Pm, Q)
+    o.poisson()
+    E, H = o.field("E"), o.field("H")
+    i0, i1 = max(0, wx0), min(Nx - 1, wx0 + m.W - 1) - 8
+    j0, j1 = max(0, wy0), min(Ny - 1, wy0 + m.H - 1) - 1
+    sl_g = (slice(j0 + 1, j1 + 2), slice(i0 + 1, i1 + 2))
+    sl_w = (slice(j0 - wy0 + 1, j1 - wy0 + 2), slice(i0 - wx0 + 1, i1 - wx0 + 2))
+    bad = []
+    for name, got, want in (("Ex", F[0], E[0]), ("Ey", F[1], E[1]), ("Hz/2", F[2], H[0]), ("Hz/2'", F[3], H[1])):
+        g, w = got[sl_w], want[sl_g]
+        scale = np.abs(w).max()
+        err = np.abs(g - w).max()
+        print("%s: max diff %.3e of %.3e" % (name, err, scale))
+        if not err <= tol * scale:
+            bad.append(name)
     assert not bad, bad

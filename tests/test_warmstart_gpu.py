@@ -1,0 +1,92 @@
+"""GPU warm start (BF16 MFMA U-Net) against the numpy restatement, and a
+short training run that must lower the loss and the multigrid V-cycles."""
+import numpy as np
+import pytest
+
+from csrst_amd import poisson as P
+from csrst_amd.warmstart import WarmStart, layers
+from tests.unet_ref import Ref
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_and_gradients_match_cpu(gpu):
+    rng = np.random.default_rng(1)
+    H, W, B = 16, 24, 2
+    u = WarmStart(H, W, max_batch=B)
+    p = u.weights()
+    p[np.concatenate([np.arange(l["boff"], l["boff"] + l["cout"]) for l in layers()[0]])] = \
+        rng.normal(0, 0.05, sum(l["cout"] for l in layers()[0]))
+    u.set_weights(p)
+    rho = rng.normal(size=(B, H, W)).astype(np.float32)
+    phi = rng.normal(size=(B, H, W)).astype(np.float32)
+    out = u.infer(rho)
+    ref = Ref(u.weights())
+    want = ref.forward(rho)
+    e = rel(out, want)
+    print("forward relative error %.2e" % e)
+    assert e < 2e-2
+    loss, g = u.gradients(rho, phi)
+    want_out = ref.forward(rho)
+    gw = ref.backward(want_out, phi)
+    assert abs(loss - np.mean((want_out - phi) ** 2)) < 2e-2 * loss
+    cos = float(g @ gw / (np.linalg.norm(g) * np.linalg.norm(gw)))
+    print("gradient cosine %.6f, relative error %.2e" % (cos, rel(g, gw)))
+    assert cos > 0.999
+    for i, l in enumerate(layers()[0]):
+        sl = slice(l["woff"], l["boff"] + l["cout"])
+        e = rel(g[sl], gw[sl])
+        assert e < 6e-2, (i, e)
+
+
+def make_sample(rng, H, W, h):
+    """1-4 Gaussians (sigma 40-500 um) summing to one charge unit, on an
+    H x W grid of spacing h inside a rectangular chamber (edges grounded)."""
+    y, x = np.mgrid[0:H, 0:W]
+    rho = np.zeros((H, W))
+    k = rng.integers(1, 5)
+    amps = rng.random(k)
+    amps /= amps.sum()
+    for a in amps:
+        s = rng.uniform(40e-6, 500e-6) / h
+        cx, cy = rng.uniform(0.25 * W, 0.75 * W), rng.uniform(0.25 * H, 0.75 * H)
+        rho += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s)) / (2 * np.pi * s * s)
+    return rho
+
+
+def test_training_saves_vcycles(gpu):
+    rng = np.random.default_rng(2)
+    H, W, h = 64, 128, 80e-6
+    cls = np.zeros((H, W), np.uint8)
+    cls[1:-1, 1:-1] = P.UNKNOWN
+    data = []
+    for _ in range(96):
+        rho = make_sample(rng, H, W, h)
+        phi, cyc, res = P.solve(h, h, cls, rho, tol=1e-12, maxit=100)
+        data.append((rho, phi))
+    scale = 1.0 / max(np.abs(r).max() for r, _ in data)
+    pscale = 1.0 / max(np.abs(p).max() for _, p in data)
+    u = WarmStart(H, W, max_batch=8)
+    train, test = data[:80], data[80:]
+    first = None
+    for epoch in range(25):
+        for b in range(0, len(train), 8):
+            rb = np.stack([r for r, _ in train[b:b + 8]])
+            pb = np.stack([p for _, p in train[b:b + 8]]) * pscale
+            loss = u.train_step(rb, pb, lr=2e-3, scale=scale)
+            first = loss if first is None else first
+    print("loss %.3e -> %.3e" % (first, loss))
+    assert loss < 0.1 * first
+    cold, warm = [], []
+    for rho, _ in test:
+        guess = u.infer(rho[None], scale=scale)[0].astype(np.float64) / pscale
+        _, c0, _ = P.solve(h, h, cls, rho, tol=1e-10, maxit=200)
+        _, c1, _ = P.solve(h, h, cls, rho, phi0=guess, tol=1e-10, maxit=200)
+        cold.append(c0)
+        warm.append(c1)
+    print("V-cycles to 1e-10: cold %.2f, warm %.2f" % (np.mean(cold), np.mean(warm)))
+    assert np.mean(warm) < np.mean(cold)
