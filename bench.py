@@ -33,7 +33,7 @@ def run_mw(cfg, fp32, steps, warmup, R, P, Q, stat):
     from csrst_amd.mw import MovingWindowStep
     m = MovingWindowStep(cfg["W"], cfg["H"], cfg["dx"], cfg["dy"], cfg["lenBefore"], cfg["lenAfter"], cfg["width"],
                          cfg["phiDeg"], cfg["ekinMeV"], cfg["bz"], order=cfg["order"], fp32=fp32,
-                         sortEvery=cfg["sortEvery"], statModulus=stat)
+                         sortEvery=cfg["sortEvery"], statModulus=stat, cfl=cfg["cfl"], pml=cfg["pml"])
     m.init_bunch(R, P, Q)
     if warmup:
         m.step(warmup)
@@ -179,7 +179,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (split int32+fp32 positions)",
         "data": "synthetic seeded Gaussian bunch (Philox), analytic bend chamber",
-        "config": {"workload": workload, "cells": cells, "particles": n},
+        "config": {"workload": workload, "cells": cells, "particles": n, "pml": cfg["pml"], "cfl": cfg["cfl"]},
         "cell_updates_per_s": cu32,
         "step_roofline_frac": frac_step,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -25,26 +25,26 @@ def config(name):
         g, R = 100.0, 1.0
         return dict(name=name, W=512, H=256, dx=50e-6, dy=50e-6, width=12.0e-3, phiDeg=10.0, gamma=g,
                     ekinMeV=ekin_mev(g), bz=bz_for(g, R), R=R, lenBefore=0.05, lenAfter=0.02, n=100000,
-                    charge_nC=1.0, sigma_s=1e-3, sigma_x=0.5e-3, espread=0.0, chirp=0.0, order=0, steps=2000,
-                    sortEvery=20)
+                    charge_nC=1.0, sigma_s=1e-3, sigma_x=0.5e-3, espread=0.0, chirp=0.0, order=1, steps=2000,
+                    sortEvery=20, cfl=0.5, pml=16, poisson=True)
     if name == "c2":
         g, R = 500.0, 5.0
         return dict(name=name, W=8192, H=2048, dx=10e-6, dy=10e-6, width=20.0e-3, phiDeg=30.0, gamma=g,
                     ekinMeV=ekin_mev(g), bz=bz_for(g, R), R=R, lenBefore=0.10, lenAfter=0.10, n=10_000_000,
                     charge_nC=0.5, sigma_s=200e-6, sigma_x=100e-6, espread=1e-3, chirp=0.0, order=2,
-                    steps=20000, sortEvery=20)
+                    steps=20000, sortEvery=20, cfl=1.0, pml=32)
     if name == "c3":
         g, R = 1000.0, 10.0
         return dict(name=name, W=32768, H=4096, dx=5e-6, dy=5e-6, width=20.0e-3, phiDeg=20.0, gamma=g,
                     ekinMeV=ekin_mev(g), bz=bz_for(g, R), R=R, lenBefore=0.20, lenAfter=0.10, n=100_000_000,
                     charge_nC=1.0, sigma_s=50e-6, sigma_x=50e-6, espread=0.0, chirp=-10.0, order=2,
-                    steps=10000, sortEvery=20)
+                    steps=10000, sortEvery=20, cfl=1.0, pml=0)
     if name == "c5":
         g, R = 300.0, 2.0
         return dict(name=name, W=16384, H=8192, dx=2e-6, dy=2e-6, width=16.0e-3, phiDeg=15.0, gamma=g,
                     ekinMeV=ekin_mev(g), bz=bz_for(g, R), R=R, lenBefore=0.10, lenAfter=0.05, n=50_000_000,
                     charge_nC=0.2, sigma_s=300e-6, sigma_x=50e-6, espread=0.0, chirp=0.0, order=2,
-                    steps=15000, sortEvery=20, modulation=(0.1, 20e-6))
+                    steps=15000, sortEvery=20, modulation=(0.1, 20e-6), cfl=1.0, pml=32, poisson=True)
     raise KeyError(name)
 
 

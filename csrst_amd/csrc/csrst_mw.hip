@@ -69,6 +69,13 @@ struct Fields {
     T *cut;              // 5 planes: lam0, lam1, area, lamN0, lamN1
     int W, H, Wp, Hp;    // logical interior size, physical (padded) size
     size_t plane;
+    // convolutional PML along x at both window ends (npml columns each):
+    // psi of the x-derivative in the Hz update (ping-pong with k_heh) and in
+    // the Ey update; per logical column: decay b and weight a = b - 1 for
+    // the Hz half step (at i + 1/2) and the Ey step (at i)
+    int npml;
+    T *psih, *psih2, *psie;
+    const T *pml;        // [4][W]: bh, ah, be, ae
 };
 
 template <class T>
@@ -125,6 +132,7 @@ __global__ void k_geom_all(Fields<T> f, Geo g, const WinState *ws)
     if (i > f.W - 1 + G) return;
     const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
     f.ex[q] = 0; f.ey[q] = 0; f.hz[q] = 0; f.jx[q] = 0; f.jy[q] = 0; f.ex2[q] = 0; f.hz2[q] = 0;
+    if (f.npml) { f.psih[q] = 0; f.psih2[q] = 0; f.psie[q] = 0; }
     cell_geometry(g, f, ws->wx0 + i, ws->wy0 + j, q);
 }
 
@@ -294,17 +302,99 @@ __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bo
     return (dI >= -1 && dI <= 1 && dJ >= -1 && dJ <= 1);
 }
 
+// Segment currents of the S1 and S2 deposits (FirstOrderShapeFunction.cpp:
+// 556-587, SecondOrderShapeFunction.cpp:583-826, the same time integrals as
+// pm::seg_s1 / pm::seg_s2) with the node weights regrouped as separable
+// sums: per node w = X[i] S[j] + C[i] P[j] - B[i] Q[j], the per-axis factors
+// formed once per segment.  Equal in exact arithmetic; fewer operations.
+template <class T>
+__device__ __forceinline__ void seg_s1_sep(pm::SegCurrent<T> &o, int comp, T tau, vec2<T> a, vec2<T> b, vec2<T> qodt,
+                                           vec2<T> d)
+{
+    const T h = T(0.5), o3 = T(1.0 / 3.0), o4 = h * h;
+    const T tp2 = tau * tau * h;
+    const T tp3 = tp2 * tau * 2 * o3;
+    // along the current: A + B (linear); across: quadratic Ay + By + Cy
+    const T al = comp == 0 ? a.x : a.y, dl = comp == 0 ? d.x : d.y;
+    const T bt = comp == 0 ? b.y : b.x, dt = comp == 0 ? d.y : d.x;
+    const T cd = (comp == 0 ? qodt.y : qodt.x) * dl;
+    const T A[2] = {al, T(1.0) - al};
+    const T B[2] = {-dl * tau, dl * tau};
+    const T Ay[3] = {h * (h + bt) * (h + bt) * tau, (T(0.75) - bt * bt) * tau, h * (h - bt) * (h - bt) * tau};
+    const T By[3] = {-(h + bt) * dt * tp2, 2 * bt * dt * tp2, (h - bt) * dt * tp2};
+    const T Cy[3] = {h * dt * dt * tp3, -dt * dt * tp3, h * dt * dt * tp3};
+    T S[3], Q[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        S[m] = cd * (Ay[m] + By[m] + Cy[m]);
+        Q[m] = cd * (h * Ay[m] + o3 * By[m] + o4 * Cy[m]);
+    }
+    if (comp == 0) {
+        o.nx = 2; o.ny = 3;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) o.w[jj * 2 + ii] = (A[ii] + B[ii]) * S[jj] - B[ii] * Q[jj];
+    } else {
+        o.nx = 3; o.ny = 2;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii) o.w[jj * 3 + ii] = (A[jj] + B[jj]) * S[ii] - B[jj] * Q[ii];
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void seg_s2_sep(pm::SegCurrent<T> &o, int comp, T tau, vec2<T> a, vec2<T> b, vec2<T> qodt,
+                                           vec2<T> d)
+{
+    const T h = T(0.5), o3 = T(1.0 / 3.0), o4 = T(0.25), o5 = T(0.2);
+    const T tp2 = tau * tau / 2;
+    const T tp3 = tp2 * tau * 2 / 3;
+    const T tp4 = tp2 * tp2;
+    // along the current: quadratic A2 + B2 + C2 in a; across: cubic in b
+    const T al = comp == 0 ? a.x : a.y, dl = comp == 0 ? d.x : d.y;
+    const T bt = comp == 0 ? b.y : b.x, dt = comp == 0 ? d.y : d.x;
+    const T cd = (comp == 0 ? qodt.y : qodt.x) * dl;
+    const T dp2 = dl * dl;
+    const T A2[3] = {h * (h + al) * (h + al), T(0.75) - al * al, h * (h - al) * (h - al)};
+    const T B2[3] = {-(h + al) * dl * tau, 2 * al * dl * tau, (h - al) * dl * tau};
+    const T C2[3] = {h * dp2 * tp2, -dp2 * tp2, h * dp2 * tp2};
+    T A3[4], B3[4], C3[4], D3[4];
+    pm::cubic_terms(bt, dt, tau, tp2, tp3, tp4, A3, B3, C3, D3);
+    T X[3], S[4], P[4], Q[4];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X[m] = A2[m] + B2[m] + C2[m];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        S[m] = cd * (D3[m] + C3[m] + B3[m] + A3[m]);
+        P[m] = cd * ((D3[m] - A3[m]) * o3 + C3[m] * o5);
+        Q[m] = cd * (B3[m] * o3 + D3[m] * o5 + C3[m] * o4 + A3[m] * h);
+    }
+    if (comp == 0) {
+        o.nx = 3; o.ny = 4;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii) o.w[jj * 3 + ii] = X[ii] * S[jj] + C2[ii] * P[jj] - B2[ii] * Q[jj];
+    } else {
+        o.nx = 4; o.ny = 3;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) o.w[jj * 4 + ii] = X[jj] * S[ii] + C2[jj] * P[ii] - B2[jj] * Q[ii];
+    }
+}
+
 constexpr int kPPT = 8;              // particles per thread in one workgroup chunk
 constexpr int kChunk = kB * kPPT;    // 2048 particles per workgroup
 constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
-constexpr int kFW = 48, kFH = 24;    // LDS field tile for the gathers (nodes)
 
 template <class T, int ORDER, int COMP>
 __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
     __shared__ T acc[kTH * kTW];
-    __shared__ T ftile[3][kFH * kFW];   // Ex, Ey, Hz around the chunk
-    __shared__ int sbox[4];          // chunk box of old cells
+    __shared__ int sbox[2];          // chunk's minimum old cell
     __shared__ int wbox[4];          // written-node box of the accumulation tile
     const int tid = threadIdx.x, lane = tid & 63;
     const int base = blockIdx.x * kChunk;
@@ -312,48 +402,22 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     const vec2<T> p0 = pm::mk((T)ws->p0[0], (T)ws->p0[1]);
     const int W = f.W, H = f.H;
     T *J = COMP == 0 ? f.jx : f.jy;
-    // ---- chunk box of old cells: anchor of the accumulation tile and extent
-    // of the field tile
-    int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
+    // ---- tile anchor: minimum old cell of the chunk
+    int mnx = INT_MAX, mny = INT_MAX;
     for (int it = 0; it < kPPT; ++it) {
         const int k = base + it * kB + tid;
-        if (k < p.n) {
-            const int cx = p.gx[k], cy = p.gy[k];
-            mnx = min(mnx, cx); mny = min(mny, cy); mxx = max(mxx, cx); mxy = max(mxy, cy);
-        }
+        if (k < p.n) { mnx = min(mnx, p.gx[k]); mny = min(mny, p.gy[k]); }
     }
     mnx = hu::wave_min(mnx); mny = hu::wave_min(mny);
-    mxx = hu::wave_max(mxx); mxy = hu::wave_max(mxy);
     if (tid == 0) {
-        sbox[0] = INT_MAX; sbox[1] = INT_MAX; sbox[2] = INT_MIN; sbox[3] = INT_MIN;
+        sbox[0] = INT_MAX; sbox[1] = INT_MAX;
         wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN;
     }
     for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
     __syncthreads();
-    if (lane == 0) {
-        atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny);
-        atomicMax(&sbox[2], mxx); atomicMax(&sbox[3], mxy);
-    }
+    if (lane == 0) { atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny); }
     __syncthreads();
     const int ax = sbox[0] - 2, ay = sbox[1] - 2;   // LDS node (0,0) = global node (ax, ay)
-    // Field tile: the gathers at the half-drifted positions read nodes within
-    // [-3, +3] cells of a particle's old cell (half drift <= 1/2 cell, S2
-    // stencils span [-2, +2] about the new cell), so old-cell box -4 .. +3.
-    // A chunk whose box does not fit (sparse tails) gathers from HBM.
-    const int tx0 = sbox[0] - 4, ty0 = sbox[1] - 4;
-    const int tw = sbox[2] - sbox[0] + 8, th = sbox[3] - sbox[1] + 8;
-    const bool tiled = sbox[0] <= sbox[2] && tw <= kFW && th <= kFH && tx0 - wx0 >= -G &&
-                       tx0 + tw - 1 - wx0 <= W - 1 + G && ty0 - wy0 >= -G && ty0 + th - 1 - wy0 <= H - 1 + G;
-    if (tiled) {
-        for (int q = tid; q < tw * th; q += kB) {
-            const int li = q % tw, lj = q / tw;
-            const size_t o = (size_t)ringy(ty0 + lj - wy0, ry, f.Hp) * f.Wp + ringx(tx0 + li - wx0, rx, f.Wp);
-            ftile[0][lj * kFW + li] = f.ex[o];
-            ftile[1][lj * kFW + li] = f.ey[o];
-            ftile[2][lj * kFW + li] = f.hz[o];
-        }
-        __syncthreads();
-    }
     int err = 0;
     int bminx = INT_MAX, bminy = INT_MAX, bmaxx = INT_MIN, bmaxy = INT_MIN;
     int tminx = INT_MAX, tminy = INT_MAX, tmaxx = INT_MIN, tmaxy = INT_MIN;
@@ -405,25 +469,33 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             // ring offsets of the stencil rows/columns computed once per field
             const vec2<T> pp = pm::mk(fx, fy);
             constexpr int N = ORDER + 2;
-            auto gat = [&](int fi, const T *plane, const pm::Stencil<T> &st) -> T {
-                if (tiled) {
-                    const T *tp = &ftile[fi][(gy + (int)st.J0 - ty0) * kFW + (gx + (int)st.I0 - tx0)];
-                    return pm::gather_n<N>(st, false, [&](int a, int b) { return tp[b * kFW + a]; });
+            // Fast path: the N x N block lies inside the window and does not
+            // straddle the ring seam, so it is N rows of N contiguous values.
+            auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
+                const int i0 = gx + (int)st.I0 - wx0, j0 = gy + (int)st.J0 - wy0;
+                int px0 = i0 + G + rx, py0 = j0 + G + ry;
+                px0 = px0 >= f.Wp ? px0 - f.Wp : px0;
+                py0 = py0 >= f.Hp ? py0 - f.Hp : py0;
+                const bool fast = i0 >= -G && i0 + N - 1 <= W - 1 + G && j0 >= -G && j0 + N - 1 <= H - 1 + G &&
+                                  px0 >= 0 && py0 >= 0 && px0 + N - 1 < f.Wp && py0 + N - 1 < f.Hp;
+                if (fast) {
+                    const T *b0 = plane + (size_t)py0 * f.Wp + px0;
+                    return pm::gather_n<N>(st, false, [&](int a, int b) { return b0[(size_t)b * f.Wp + a]; });
                 }
                 int col[N];
                 size_t rowb[N];
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
-                    const int i = gx + (int)st.I0 + a - wx0, j = gy + (int)st.J0 + a - wy0;
+                    const int i = i0 + a, j = j0 + a;
                     if (i < -G || i > W - 1 + G || j < -G || j > H - 1 + G) err |= 1;
                     col[a] = ringx(min(max(i, -G), W - 1 + G), rx, f.Wp);
                     rowb[a] = (size_t)ringy(min(max(j, -G), H - 1 + G), ry, f.Hp) * f.Wp;
                 }
                 return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
             };
-            const T ex = gat(0, f.ex, pm::edge_stencil(ORDER, 0, pp));
-            const T ey = gat(1, f.ey, pm::edge_stencil(ORDER, 1, pp));
-            const T hf = gat(2, f.hz, pm::cell_stencil(ORDER, pp));
+            const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
+            const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
+            const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
             const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
             const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
             const T bext = (T)geom::external_bz(g.bend, X, Y);
@@ -460,8 +532,8 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
                     const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
                     const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
                     const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
-                    if (ORDER == 1) pm::seg_s1(sc, COMP, tau, a, b, qodt, delta);
-                    else pm::seg_s2(sc, COMP, tau, a, b, qodt, delta);
+                    if (ORDER == 1) seg_s1_sep(sc, COMP, tau, a, b, qodt, delta);
+                    else seg_s2_sep(sc, COMP, tau, a, b, qodt, delta);
                     sc.I0 = I - 1; sc.J0 = Jn - 1;
                 }
                 sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
@@ -609,11 +681,15 @@ __device__ __forceinline__ T ex_update(const Fields<T> &f, size_t q, uint8_t cls
 // of the strip rows are stored, into the partner buffers (ex2, hz2), so the
 // halo rows other strips read are never overwritten during the pass.  Guard
 // rows (-1, H) are never updated.
-template <class T>
-__global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const WinState *ws, int jbase)
+// Columns: PML = false sweeps the ncol columns from cbase; PML = true the two
+// end layers (t < npml: column t, else column W - 2 npml + t).
+template <class T, bool PML>
+__global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const WinState *ws, int jbase, int cbase,
+                                            int ncol)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // logical column 0..W-1
-    if (i >= f.W) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    const int i = PML ? (t < f.npml ? t : f.W - 2 * f.npml + t) : cbase + t;   // logical column
     const int rx = ws->rx, ry = ws->ry;
     const int j0 = jbase + blockIdx.y * kStrip;
     if (j0 > f.H - 1) return;
@@ -623,6 +699,32 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
     const int jhi0 = -ws->jbox[0][2] - ws->wx0, jhi1 = -ws->jbox[0][3] - ws->wy0;
     const bool colInJ = i >= jlo0 && i <= jhi0;
     auto row = [&](int j) { return (size_t)ringy(j, ry, f.Hp) * f.Wp; };
+    // PML column: Hz's x term (eyp - ey) becomes (eyp - ey) + psi with
+    // psi <- b psi + a (eyp - ey) at each half step (Ey does not change
+    // between the two half steps, so both use the same difference)
+    const bool inPml = PML && (i < f.npml || i >= f.W - f.npml);
+    const T bh = inPml ? f.pml[i] : T(0), ah = inPml ? f.pml[f.W + i] : T(0);
+    // x difference of the Hz update: Ey(i+1) - Ey(i) in bulk cells, the
+    // edge-length weighted n1 Ey(i+1) - l1 Ey(i) in cut cells
+    // (TETMSolver.cpp:87-104); psi_in -> psi_out is its PML convolution.
+    auto hup = [&](size_t q, uint8_t cl, T h, T ey, T eyp, T ex, T exn, T psi_in, T &psi_out) -> T {
+        if (!(PML && inPml)) return h_update(f, q, cl, h, ey, eyp, ex, exn, c);
+        if (cl == 1) {
+            const T X = eyp - ey;
+            psi_out = bh * psi_in + ah * X;
+            return (h - c.sx * (X + psi_out)) + c.sy * (exn - ex);
+        }
+        if (cl == 2) {
+            const T l0 = f.cut[q], l1 = f.cut[f.plane + q], ar = f.cut[2 * f.plane + q];
+            const T n0 = f.cut[3 * f.plane + q], n1 = f.cut[4 * f.plane + q];
+            const T X = n1 * eyp - l1 * ey;
+            psi_out = bh * psi_in + ah * X;
+            const T u0 = -c.cx * (X + psi_out) / ar;
+            const T u1 = +c.cy * (n0 * exn - l0 * ex) / ar;
+            return h + (u0 + u1);
+        }
+        return h;
+    };
     // prologue: H1(j0-1)
     T h1p;
     {
@@ -630,12 +732,14 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         const size_t rb = row(j);
         const T hold = f.hz[rb + px];
         if (j >= 0) {
-            h1p = h_update(f, rb + px, f.cls[rb + px], hold, f.ey[rb + px], f.ey[rb + pxp], f.ex[rb + px],
-                           f.ex[row(j0) + px], c);
+            T psi1 = 0;
+            h1p = hup(rb + px, f.cls[rb + px], hold, f.ey[rb + px], f.ey[rb + pxp], f.ex[rb + px],
+                      f.ex[row(j0) + px], inPml ? f.psih[rb + px] : T(0), psi1);
         } else {
             h1p = hold;
         }
     }
+    T psi1p = 0;
     T exo = f.ex[row(j0) + px];    // Ex old at the current row
     T exnp = 0, eyp_prev = 0, ey_prev = 0, h1pp = 0;
     uint8_t clp = 0;
@@ -646,9 +750,9 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         const T exo_next = f.ex[row(j + 1) + px];
         const T ey = f.ey[q], eyp = f.ey[rb + pxp];
         const uint8_t cl = f.cls[q];
-        T h1, exn;
+        T h1, exn, psi1 = 0;
         if (j <= f.H - 1) {
-            h1 = h_update(f, q, cl, f.hz[q], ey, eyp, exo, exo_next, c);
+            h1 = hup(q, cl, f.hz[q], ey, eyp, exo, exo_next, inPml ? f.psih[q] : T(0), psi1);
             const T jv = (colInJ && j >= jlo1 && j <= jhi1) ? f.jx[q] : T(0);
             exn = ex_update(f, q, cl, exo, h1, h1p, jv, c);
         } else {
@@ -657,9 +761,12 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         }
         if (j > j0) {
             // H2(j-1) from H1(j-1), Ex new at j-1 and j
-            f.hz2[qprev] = h_update(f, qprev, clp, h1pp, ey_prev, eyp_prev, exnp, exn, c);
+            T psi2 = 0;
+            f.hz2[qprev] = hup(qprev, clp, h1pp, ey_prev, eyp_prev, exnp, exn, psi1p, psi2);
+            if (inPml) f.psih2[qprev] = psi2;
             f.ex2[qprev] = exnp;
         }
+        psi1p = psi1;
         h1pp = h1;
         h1p = h1;
         exnp = exn;
@@ -671,11 +778,12 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
 // Ey (TETMSolver.hh:241-249, conformal TETMSolver.cpp:117-127); clears Jy in
 // its footprint after use and Jx in the Jx footprint, leaving both current
 // planes zero for the next deposits.
-template <class T>
-__global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const WinState *ws)
+template <class T, bool PML>
+__global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const WinState *ws, int cbase, int ncol)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.W) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    const int i = PML ? (t < f.npml ? t : f.W - 2 * f.npml + t) : cbase + t;
     const int rx = ws->rx, ry = ws->ry;
     const int px = ringx(i, rx, f.Wp), pxm = ringx(i - 1, rx, f.Wp);
     const int jlo0 = ws->jbox[1][0] - ws->wx0, jlo1 = ws->jbox[1][1] - ws->wy0;
@@ -684,6 +792,8 @@ __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const W
     const int xlo0 = ws->jbox[0][0] - ws->wx0, xlo1 = ws->jbox[0][1] - ws->wy0;
     const int xhi0 = -ws->jbox[0][2] - ws->wx0, xhi1 = -ws->jbox[0][3] - ws->wy0;
     const bool colInJx = i >= xlo0 && i <= xhi0;
+    const bool inPml = PML && (i < f.npml || i >= f.W - f.npml);
+    const T be = inPml ? f.pml[2 * f.W + i] : T(0), ae = inPml ? f.pml[3 * f.W + i] : T(0);
     const int j0 = blockIdx.y * kStrip, j1 = min(j0 + kStrip - 1, f.H - 1);
     for (int j = j0; j <= j1; ++j) {
         const size_t rb = (size_t)ringy(j, ry, f.Hp) * f.Wp;
@@ -693,9 +803,23 @@ __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const W
         T jv = 0;
         if (colInJ && j >= jlo1 && j <= jhi1) { jv = f.jy[q]; f.jy[q] = 0; }
         if (colInJx && j >= xlo1 && j <= xhi1) f.jx[q] = 0;
-        if (cl == 1) f.ey[q] = f.ey[q] - (c.sey * d + c.dteps * jv);
-        else if (cl == 2 && f.cut[f.plane + q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea))
-            f.ey[q] = f.ey[q] + c.ce * (-d * c.ihx - jv);
+        if (cl == 1) {
+            T dd = d;
+            if (PML && inPml) {
+                const T psi = be * f.psie[q] + ae * d;
+                f.psie[q] = psi;
+                dd = d + psi;
+            }
+            f.ey[q] = f.ey[q] - (c.sey * dd + c.dteps * jv);
+        } else if (cl == 2 && f.cut[f.plane + q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea)) {
+            T dd = d;
+            if (PML && inPml) {
+                const T psi = be * f.psie[q] + ae * d;
+                f.psie[q] = psi;
+                dd = d + psi;
+            }
+            f.ey[q] = f.ey[q] + c.ce * (-dd * c.ihx - jv);
+        }
     }
 }
 
@@ -738,11 +862,18 @@ __global__ void k_fresh(Fields<T> f, Geo g, const WinState *ws)
     } else if (t < ncol + nrow) {
         if (mj == 0) return;
         i = t - ncol - G; j = mj > 0 ? f.H - 1 + G : -G;
+    } else if (t < 2 * ncol + nrow) {
+        // the column that moved out of the leading PML into the interior
+        if (mi == 0 || f.npml == 0) return;
+        const size_t q = (size_t)ringy(t - ncol - nrow - G, ws->ry, f.Hp) * f.Wp + ringx(f.W - f.npml - 1, ws->rx, f.Wp);
+        f.psih[q] = 0; f.psih2[q] = 0; f.psie[q] = 0;
+        return;
     } else {
         return;
     }
     const size_t q = (size_t)ringy(j, ws->ry, f.Hp) * f.Wp + ringx(i, ws->rx, f.Wp);
     f.ex[q] = 0; f.ey[q] = 0; f.hz[q] = 0; f.jx[q] = 0; f.jy[q] = 0; f.ex2[q] = 0; f.hz2[q] = 0;
+    if (f.npml) { f.psih[q] = 0; f.psih2[q] = 0; f.psie[q] = 0; }
     cell_geometry(g, f, ws->wx0 + i, ws->wy0 + j, q);
 }
 
@@ -1053,7 +1184,9 @@ struct csrst_mw {
     double lastMs[5] = {0, 0, 0, 0, 0};
     bool ready = false;
 
-    int cur = 0;   // which of the Ex/Hz ping-pong buffers is current
+    int cur = 0;   // which of the Ex/Hz (and PML psi) ping-pong buffers is current
+    int npml = 0;  // PML columns at each window end
+    DBuf<unsigned char> pmlBuf;   // psih, psih', psie planes + [4][W] coefficients
 
     template <class T>
     mw::Fields<T> fields()
@@ -1067,6 +1200,12 @@ struct csrst_mw {
         f.cut = b + 7 * plane;
         f.cls = cls.p;
         f.W = W; f.H = H; f.Wp = Wp; f.Hp = Hp; f.plane = plane;
+        f.npml = npml;
+        T *pb = reinterpret_cast<T *>(pmlBuf.p);
+        f.psih = npml ? (cur ? pb + plane : pb) : nullptr;
+        f.psih2 = npml ? (cur ? pb : pb + plane) : nullptr;
+        f.psie = npml ? pb + 2 * plane : nullptr;
+        f.pml = npml ? pb + 3 * plane : nullptr;
         return f;
     }
     template <class T>
@@ -1182,7 +1321,10 @@ void one_step(csrst_mw *m, bool timed)
     mw::SweepConst<T> sc;
     sweep_consts(m, sc);
     const mw::Fields<T> f = m->fields<T>();
-    const dim3 gs((m->W + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
+    // interior columns without PML terms; the two PML layers in a launch of their own
+    const int np = m->npml, nin = m->W - 2 * np;
+    const dim3 gs((nin + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
+    const dim3 gp((2 * np + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
     if (m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0) {
         if (timed) HU_CHECK(hipEventRecord(m->ev[6], m->s));
         sort_particles<T>(m);
@@ -1191,20 +1333,22 @@ void one_step(csrst_mw *m, bool timed)
     if (timed) HU_CHECK(hipEventRecord(m->ev[0], m->s));
     if (m->n > 0) push_dispatch<T, 0>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[1], m->s));
-    hipLaunchKernelGGL((mw::k_heh<T>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0);
+    hipLaunchKernelGGL((mw::k_heh<T, false>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, np, nin);
+    if (np) hipLaunchKernelGGL((mw::k_heh<T, true>), gp, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, 0, 2 * np);
     HU_LAUNCH_CHECK();
     m->cur ^= 1;
     const mw::Fields<T> f2 = m->fields<T>();
     if (timed) HU_CHECK(hipEventRecord(m->ev[2], m->s));
     if (m->n > 0) push_dispatch<T, 1>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[3], m->s));
-    hipLaunchKernelGGL((mw::k_ey<T>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p);
+    hipLaunchKernelGGL((mw::k_ey<T, false>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, np, nin);
+    if (np) hipLaunchKernelGGL((mw::k_ey<T, true>), gp, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, 0, 2 * np);
     HU_LAUNCH_CHECK();
     if (m->n > 0 && m->p.statModulus > 0 && m->steps % m->p.statModulus == m->p.statModulus - 1) wake<T>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[4], m->s));
     if (m->n > 0) {
         mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, m->lead);
-        const int nf = m->H + 2 * mw::G + m->W + 2 * mw::G;
+        const int nf = 2 * (m->H + 2 * mw::G) + m->W + 2 * mw::G;
         hipLaunchKernelGGL((mw::k_fresh<T>), dim3((nf + mw::kB - 1) / mw::kB), dim3(mw::kB), 0, m->s, f2, m->geo, m->ws.p);
         HU_LAUNCH_CHECK();
     }
@@ -1327,6 +1471,35 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         const size_t rs = m->f32 ? 4 : 8;
         m->fieldsBuf.alloc(m->plane * 12 * rs);   // ex ey hz jx jy ex' hz' + 5 cut planes
         m->fieldsBuf.zero(m->s);
+        if (p->pml < 0 || 2 * p->pml > p->W - 8) throw Error(CSRST_EINVAL, "pml must be in [0, (W - 8) / 2]");
+        m->npml = p->pml;
+        if (m->npml) {
+            // Convolutional PML (kappa 1, alpha 0): psi <- b psi + (b - 1) D with
+            // b = exp(-sigma dt' / eps0), sigma = sigma_max d^3 over the layer,
+            // sigma_max eta0 hx = 3.2 (the polynomial-grading optimum 0.8 (m + 1)).
+            // dt' is half a step for Hz (two half steps per step) and one step for Ey.
+            const int W = p->W, np = m->npml;
+            std::vector<double> co(4 * (size_t)W, 0.0);
+            auto depth = [&](double x) {   // x in cells (logical), depth into the layer in [0, 1]
+                if (x < np) return (np - x) / np;
+                if (x > W - np) return (x - (W - np)) / np;
+                return 0.0;
+            };
+            for (int i = 0; i < W; ++i) {
+                const double dh = depth(i + 0.5), de = depth((double)i);
+                const double bh = exp(-3.2 * dh * dh * dh * cfl * 0.5), be = exp(-3.2 * de * de * de * cfl);
+                co[i] = bh; co[W + i] = bh - 1; co[2 * (size_t)W + i] = be; co[3 * (size_t)W + i] = be - 1;
+            }
+            m->pmlBuf.alloc((3 * m->plane + 4 * (size_t)W) * rs);
+            m->pmlBuf.zero(m->s);
+            HU_CHECK(hipStreamSynchronize(m->s));
+            if (m->f32) {
+                std::vector<float> cf(co.begin(), co.end());
+                HU_CHECK(hipMemcpy(m->pmlBuf.p + 3 * m->plane * rs, cf.data(), cf.size() * 4, hipMemcpyHostToDevice));
+            } else {
+                HU_CHECK(hipMemcpy(m->pmlBuf.p + 3 * m->plane * rs, co.data(), co.size() * 8, hipMemcpyHostToDevice));
+            }
+        }
         m->cls.alloc(m->plane);
         m->cls.zero(m->s);
         m->ws.alloc(1);

@@ -1447,6 +1447,36 @@ int csrst_poisson_compute_field(csrst_ctx *c, double tol, int maxit, int *cycles
     }
 }
 
+int csrst_substep(csrst_ctx *c, int op)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "not initialised");
+        switch (op) {
+        case 0: push(c, 0); break;
+        case 1: push(c, 1); break;
+        case 2: h_half(c); break;
+        case 3: e_x(c); break;
+        case 4: e_y(c); break;
+        case 5: exchange(c, 0); break;
+        case 6: exchange(c, 1); break;
+        case 7: exchange(c, 2); break;
+        case 8: sc_hz(c); break;
+        case 9: sc_ex(c); break;
+        case 10: sc_ey(c); break;
+        case 11: drift(c, -0.5 * c->dt); break;
+        case 12: wake_sample(c); break;
+        case 13: c->gJ.zero(c->s); break;
+        case 14: move_mesh(c, nullptr, nullptr); ++c->steps; break;
+        default: throw Error(CSRST_EINVAL, hu::fmt("unknown substep %d", op));
+        }
+        HU_CHECK(hipStreamSynchronize(c->s));
+        check_err(c);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
 int csrst_step(csrst_ctx *c, int nsteps)
 {
     try {

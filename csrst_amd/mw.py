@@ -15,10 +15,10 @@ C_LIGHT = 299792458.0
 
 class MovingWindowStep:
     def __init__(self, W, H, dx, dy, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order=2, fp32=True,
-                 sortEvery=20, statModulus=100, cfl=1.0):
+                 sortEvery=20, statModulus=100, cfl=1.0, pml=0):
         self.W, self.H = W, H
         self._p = _lib.MWParams(W, H, dx, dy, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order,
-                                1 if fp32 else 0, sortEvery, statModulus, cfl)
+                                1 if fp32 else 0, sortEvery, statModulus, cfl, pml)
         h = C.c_void_p()
         check(_lib.lib().csrst_mw_create(C.byref(self._p), C.byref(h)))
         self.h = h

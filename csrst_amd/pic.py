@@ -113,6 +113,10 @@ class TwoGridStep:
     def updateField_secondStep(self):
         check(_lib.lib().csrst_second_step(self.h), self.h)
 
+    def substep(self, op):
+        """One part of the step (codes of csrst_substep), for bisection against the oracle."""
+        check(_lib.lib().csrst_substep(self.h, C.c_int(op)), self.h)
+
     def moveMesh(self):
         mi, mj = C.c_int(), C.c_int()
         check(_lib.lib().csrst_move_window(self.h, C.byref(mi), C.byref(mj)), self.h)

@@ -83,6 +83,12 @@ int csrst_step(csrst_ctx *c, int nsteps);
 int csrst_first_step(csrst_ctx *c);            /* TETMSolver.hh:110-211 */
 int csrst_second_step(csrst_ctx *c);           /* TETMSolver.hh:213-266 */
 int csrst_move_window(csrst_ctx *c, int *moveI, int *moveJ);   /* main.cpp:494-501 */
+/* One part of a step, in the order of TETMSolver.hh:110-266 and main.cpp:494-501:
+ * 0 push + deposit Jx, 1 push + deposit Jy (PartBunch::push_particles),
+ * 2 H half step, 3 Ex, 4 Ey (global and window sweeps), 5/6/7 exchange H/Ex/Ey
+ * (exchangeBoundaries), 8/9/10 conformal Hz/Ex/Ey (SCBoundaryCondition),
+ * 11 drift -dt/2, 12 wake sample, 13 J = 0, 14 moveMesh. */
+int csrst_substep(csrst_ctx *c, int op);
 
 /* which: 0 global E, 1 global H, 2 window E, 3 window H, 4 window J, 5 global J.
  * Planar layout [2][m+2][n+2] over [-1, n] x [-1, m] (guard layer included).
@@ -108,6 +114,8 @@ typedef struct {
     int sortEvery;           /* cell sort period in steps (0: never) */
     int statModulus;         /* wake sampling period */
     double cfl;              /* c*dt/dx; 1.0 is the reference's dt = hx/c */
+    int pml;                 /* absorbing-layer columns at each window end (0: none, as the
+                                reference, whose PML coefficients are zeroed, TETMSolver.cpp:59-62) */
 } csrst_mw_params;
 
 int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out);

@@ -1832,6 +1832,34 @@ struct Sim {
         return 0;
     }
 
+    /* One part of the step (the order of step() below), for bisection:
+     * 0 push Jx, 1 push Jy, 2 H half step, 3 Ex, 4 Ey, 5 exchange H,
+     * 6 exchange Ex, 7 exchange Ey, 8 SC Hz, 9 SC Ex, 10 SC Ey,
+     * 11 drift -dt/2, 12 wake sample, 13 J = 0, 14 moveMesh (+ step count). */
+    int substep(int op)
+    {
+        const int pos[2] = {ddim[0][0], ddim[1][0]};
+        switch (op) {
+        case 0: push(0); break;
+        case 1: push(1); break;
+        case 2: h_half(); break;
+        case 3: e_x(); break;
+        case 4: e_y(); break;
+        case 5: exchange(OP_H, pos); break;
+        case 6: exchange(OP_EX, pos); break;
+        case 7: exchange(OP_EY, pos); break;
+        case 8: sc_hz(); break;
+        case 9: sc_ex(); break;
+        case 10: sc_ey(); break;
+        case 11: drift(-0.5 * dt); break;
+        case 12: wake_sample(); break;
+        case 13: J.c[0].assign(J.c[0].size(), 0.0); J.c[1].assign(J.c[1].size(), 0.0); break;
+        case 14: move_mesh(); ++steps; break;
+        default: return -1;
+        }
+        return 0;
+    }
+
     /* main.cpp:390-506, one iteration */
     void step()
     {
@@ -2151,6 +2179,15 @@ int orc_sim_poisson(void *s, double gamma)
         return as<Sim>(s)->poisson(gamma);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "oracle poisson: %s\n", e.what());
+        return -1;
+    }
+}
+int orc_sim_substep(void *s, int op)
+{
+    try {
+        return as<Sim>(s)->substep(op);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle substep: %s\n", e.what());
         return -1;
     }
 }

@@ -96,6 +96,8 @@ int   orc_sim_set_field(void *s, int which, const double *in);
 /* Global-grid sweeps only (no particles, no window): per step
  * H/2, SC Hz, Ex, SC Ex, H/2, SC Hz, Ey, SC Ey with J as set. */
 int   orc_sim_field_steps(void *s, int nsteps);
+/* one part of the step (codes as csrst_substep of include/csrst_hip.h) */
+int   orc_sim_substep(void *s, int op);
 
 /* ---- Poisson initialisation (PoissonSolver.cpp:85-201, 678-766) ------ */
 /* Node classes of cutoffStencil on the bend's Nx x Ny grid (row-major);

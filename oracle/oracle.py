@@ -236,6 +236,10 @@ class Sim:
         if rc != 0:
             raise RuntimeError("oracle poisson failed (%d)" % rc)
 
+    def substep(self, op):
+        if lib().orc_sim_substep(C.c_void_p(self.h), C.c_int(op)) != 0:
+            raise RuntimeError("oracle substep %d failed" % op)
+
     def step(self, n=1):
         rc = lib().orc_sim_step(C.c_void_p(self.h), C.c_int(n))
         if rc != 0:

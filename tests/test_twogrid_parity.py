@@ -71,3 +71,20 @@ def test_wake_and_diag(gpu):
     dg, do = g.diag(), o.diag()
     assert np.allclose(dg[:5], do[:5], rtol=1e-13, atol=0)
     assert abs(dg[5] - do[5]) <= 1e-10 * abs(do[5])
+
+
+SUBSTEPS = [0, 2, 5, 8, 3, 6, 9, 2, 5, 8, 11, 1, 4, 7, 10, 11, 12, 13, 14]
+
+
+@pytest.mark.parametrize("order", [0, 2])
+def test_substeps_bit_identical(gpu, order):
+    """The step taken apart (csrst_substep) against the oracle's parts, every
+    field compared bit for bit after every part of two steps."""
+    g, o = make(order, n=1000, stat=1)
+    for step in range(2):
+        for op in SUBSTEPS:
+            g.substep(op)
+            o.substep(op)
+            for f in ("E", "H", "mE", "mH", "mJ"):
+                assert same(g.field(f), o.field(f)), "step %d, part %d: field %s differs" % (step, op, f)
+    assert g.info()["pos"] == o.info()["pos"]
