@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libcsrst.so")
+# CSRST_LIB selects another build of the library (for comparing variants)
+LIBPATH = os.environ.get("CSRST_LIB") or os.path.join(HERE, "libcsrst.so")
 _LIB = None
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EWINDOW", -5: "EGEOM", -6: "ESTATE"}

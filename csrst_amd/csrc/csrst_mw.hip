@@ -34,7 +34,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/csrst_hip.h"
+#include "csrst_hip.h"
 #include "bend_geom.h"
 #include "hip_util.h"
 #include "pic_math.h"

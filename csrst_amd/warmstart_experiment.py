@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--seconds", type=float, default=150.0, help="training time budget")
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--seed", type=int, default=7)
     a = ap.parse_args()
     sys.path.insert(0, __file__.rsplit("/csrst_amd/", 1)[0])

@@ -154,34 +154,30 @@ def test_first_step_ex_is_deposit(gpu, order):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_pml_absorbs_outgoing_pulse(gpu, fp32):
     """A TEM pulse (Ey = Z0 Hz, travelling +x) between the walls of the
-    straight chamber section: with 16 PML columns at the window ends the field
-    energy left after the pulse has run into the leading end is a small
-    fraction of what the bare window (a reflecting end) keeps."""
+    straight chamber section runs into the window's leading end.  Without a
+    PML the end reflects it (the guard column is a conductor); with 16 PML
+    columns the wave coming back is at the percent level or below.  Measured
+    as the peak |Ey| in the band the reflected pulse then crosses."""
     b = O.Bend(Nx=NX, Ny=NY, **GEOM)
     hx, hy = b.hr
     W, H = 200, 56
     Z0 = np.sqrt(1.256637061e-06 / 8.854187817e-12)
-    left = {}
+    back = {}
     for pml in (0, 16):
         m = MovingWindowStep(W, H, hx, hy, order=0, fp32=fp32, sortEvery=0, statModulus=0, pml=pml, **GEOM)
         m.place(10, -2)                       # rows -2..53 hold the chamber (walls at rows 1 and 51)
         F = np.zeros((6, H + 2, W + 2))
         i = np.arange(W)
-        g = lambda x: np.exp(-0.5 * ((x - 100.0) / 6.0) ** 2)
+        g = lambda x: np.exp(-0.5 * ((x - 60.0) / 6.0) ** 2)
         inside = slice(1 + 3, 1 + 3 + 50)      # logical rows 3..52 = global rows 1..50
         F[1][inside, 1:-1] = g(i)[None, :]
         F[2][inside, 1:-1] = 0.5 * g(i + 0.5)[None, :] / Z0
         F[3][inside, 1:-1] = F[2][inside, 1:-1]
         m.set_fields(F)
-        F0 = m.fields()
-        energy = lambda A: np.sum(A[0] ** 2 + A[1] ** 2) + Z0 ** 2 * np.sum((A[2] + A[3]) ** 2)
-        e0 = energy(F0)
-        m.step(40)
-        mid = energy(m.fields())
-        m.step(460)
-        left[pml] = energy(m.fields()) / e0
-        print("pml %d: energy %.3e after 40 steps, %.3e after 500" % (pml, mid / e0, left[pml]))
-        assert 0.9 < mid / e0 < 1.1            # still inside the window, energy conserved
+        m.step(200)                            # reflected pulse (if any) centred near column 138
+        Ey = m.fields()[1][inside, 1:-1]
+        back[pml] = np.abs(Ey[:, 110:181]).max()
+        print("pml %d: peak |Ey| coming back %.3e" % (pml, back[pml]))
         m.close()
-    assert left[0] > 0.5
-    assert left[16] < 0.01 * left[0]
+    assert back[0] > 0.5
+    assert back[16] < 0.02

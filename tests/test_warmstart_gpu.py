@@ -78,7 +78,7 @@ def test_training_saves_vcycles(gpu):
         for b in range(0, len(train), 8):
             rb = np.stack([r for r, _ in train[b:b + 8]])
             pb = np.stack([p for _, p in train[b:b + 8]]) * pscale
-            losses.append(u.train_step(rb, pb, lr=1e-3, scale=scale))
+            losses.append(u.train_step(rb, pb, lr=2e-4, scale=scale))
     first, last = np.mean(losses[:10]), np.mean(losses[-10:])
     print("loss %.3e -> %.3e" % (first, last))
     assert last < 0.2 * first
