@@ -700,25 +700,28 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
     const bool colInJ = i >= jlo0 && i <= jhi0;
     auto row = [&](int j) { return (size_t)ringy(j, ry, f.Hp) * f.Wp; };
     // PML column: Hz's x term (eyp - ey) becomes (eyp - ey) + psi with
-    // psi <- b psi + a (eyp - ey) at each half step (Ey does not change
-    // between the two half steps, so both use the same difference)
+    // psi <- b psi + a (eyp - ey)
     const bool inPml = PML && (i < f.npml || i >= f.W - f.npml);
     const T bh = inPml ? f.pml[i] : T(0), ah = inPml ? f.pml[f.W + i] : T(0);
     // x difference of the Hz update: Ey(i+1) - Ey(i) in bulk cells, the
     // edge-length weighted n1 Ey(i+1) - l1 Ey(i) in cut cells
     // (TETMSolver.cpp:87-104); psi_in -> psi_out is its PML convolution.
-    auto hup = [&](size_t q, uint8_t cl, T h, T ey, T eyp, T ex, T exn, T psi_in, T &psi_out) -> T {
+    // The x difference is the same in both half steps (Ey is not updated in
+    // between), so psi advances once per step (update = true in the first
+    // half step) and both halves use it: the H change over the step is then
+    // that of the one-step convolutional PML.
+    auto hup = [&](size_t q, uint8_t cl, T h, T ey, T eyp, T ex, T exn, T psi_in, T &psi_out, bool update) -> T {
         if (!(PML && inPml)) return h_update(f, q, cl, h, ey, eyp, ex, exn, c);
         if (cl == 1) {
             const T X = eyp - ey;
-            psi_out = bh * psi_in + ah * X;
+            psi_out = update ? bh * psi_in + ah * X : psi_in;
             return (h - c.sx * (X + psi_out)) + c.sy * (exn - ex);
         }
         if (cl == 2) {
             const T l0 = f.cut[q], l1 = f.cut[f.plane + q], ar = f.cut[2 * f.plane + q];
             const T n0 = f.cut[3 * f.plane + q], n1 = f.cut[4 * f.plane + q];
             const T X = n1 * eyp - l1 * ey;
-            psi_out = bh * psi_in + ah * X;
+            psi_out = update ? bh * psi_in + ah * X : psi_in;
             const T u0 = -c.cx * (X + psi_out) / ar;
             const T u1 = +c.cy * (n0 * exn - l0 * ex) / ar;
             return h + (u0 + u1);
@@ -734,7 +737,7 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         if (j >= 0) {
             T psi1 = 0;
             h1p = hup(rb + px, f.cls[rb + px], hold, f.ey[rb + px], f.ey[rb + pxp], f.ex[rb + px],
-                      f.ex[row(j0) + px], inPml ? f.psih[rb + px] : T(0), psi1);
+                      f.ex[row(j0) + px], inPml ? f.psih[rb + px] : T(0), psi1, true);
         } else {
             h1p = hold;
         }
@@ -752,7 +755,7 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         const uint8_t cl = f.cls[q];
         T h1, exn, psi1 = 0;
         if (j <= f.H - 1) {
-            h1 = hup(q, cl, f.hz[q], ey, eyp, exo, exo_next, inPml ? f.psih[q] : T(0), psi1);
+            h1 = hup(q, cl, f.hz[q], ey, eyp, exo, exo_next, inPml ? f.psih[q] : T(0), psi1, true);
             const T jv = (colInJ && j >= jlo1 && j <= jhi1) ? f.jx[q] : T(0);
             exn = ex_update(f, q, cl, exo, h1, h1p, jv, c);
         } else {
@@ -762,7 +765,7 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         if (j > j0) {
             // H2(j-1) from H1(j-1), Ex new at j-1 and j
             T psi2 = 0;
-            f.hz2[qprev] = hup(qprev, clp, h1pp, ey_prev, eyp_prev, exnp, exn, psi1p, psi2);
+            f.hz2[qprev] = hup(qprev, clp, h1pp, ey_prev, eyp_prev, exnp, exn, psi1p, psi2, false);
             if (inPml) f.psih2[qprev] = psi2;
             f.ex2[qprev] = exnp;
         }
@@ -1474,10 +1477,10 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         if (p->pml < 0 || 2 * p->pml > p->W - 8) throw Error(CSRST_EINVAL, "pml must be in [0, (W - 8) / 2]");
         m->npml = p->pml;
         if (m->npml) {
-            // Convolutional PML (kappa 1, alpha 0): psi <- b psi + (b - 1) D with
-            // b = exp(-sigma dt' / eps0), sigma = sigma_max d^3 over the layer,
-            // sigma_max eta0 hx = 3.2 (the polynomial-grading optimum 0.8 (m + 1)).
-            // dt' is half a step for Hz (two half steps per step) and one step for Ey.
+            // Convolutional PML (kappa 1, alpha 0): psi <- b psi + (b - 1) D once
+            // per step, b = exp(-sigma dt / eps0), sigma = sigma_max d^3 over the
+            // layer with sigma_max eta0 hx = 1.6 (the lowest reflection of a 1-D
+            // scan of the grading for this scheme at CFL 1 and 0.5).
             const int W = p->W, np = m->npml;
             std::vector<double> co(4 * (size_t)W, 0.0);
             auto depth = [&](double x) {   // x in cells (logical), depth into the layer in [0, 1]
@@ -1487,7 +1490,7 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
             };
             for (int i = 0; i < W; ++i) {
                 const double dh = depth(i + 0.5), de = depth((double)i);
-                const double bh = exp(-3.2 * dh * dh * dh * cfl * 0.5), be = exp(-3.2 * de * de * de * cfl);
+                const double bh = exp(-1.6 * dh * dh * dh * cfl), be = exp(-1.6 * de * de * de * cfl);
                 co[i] = bh; co[W + i] = bh - 1; co[2 * (size_t)W + i] = be; co[3 * (size_t)W + i] = be - 1;
             }
             m->pmlBuf.alloc((3 * m->plane + 4 * (size_t)W) * rs);

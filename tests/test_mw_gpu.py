@@ -158,14 +158,14 @@ def test_pml_absorbs_outgoing_pulse(gpu, fp32):
     PML the end reflects it (the guard column is a conductor); with 16 PML
     columns the wave coming back is at the percent level or below.  Measured
     as the peak |Ey| in the band the reflected pulse then crosses."""
-    b = O.Bend(Nx=NX, Ny=NY, **GEOM)
-    hx, hy = b.hr
+    hy = GEOM["width"] / 50                    # walls on grid lines: the TEM mode is exact
+    hx = 0.8 * hy
     W, H = 200, 56
     Z0 = np.sqrt(1.256637061e-06 / 8.854187817e-12)
     back = {}
     for pml in (0, 16):
         m = MovingWindowStep(W, H, hx, hy, order=0, fp32=fp32, sortEvery=0, statModulus=0, pml=pml, **GEOM)
-        m.place(10, -2)                       # rows -2..53 hold the chamber (walls at rows 1 and 51)
+        m.place(10, -2)                       # rows -2..53 hold the chamber (cell rows 1..50)
         F = np.zeros((6, H + 2, W + 2))
         i = np.arange(W)
         g = lambda x: np.exp(-0.5 * ((x - 60.0) / 6.0) ** 2)
@@ -180,4 +180,4 @@ def test_pml_absorbs_outgoing_pulse(gpu, fp32):
         print("pml %d: peak |Ey| coming back %.3e" % (pml, back[pml]))
         m.close()
     assert back[0] > 0.5
-    assert back[16] < 0.02
+    assert back[16] < 5e-3

