@@ -396,6 +396,7 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     __shared__ T acc[kTH * kTW];
     __shared__ int sbox[2];          // chunk's minimum old cell
     __shared__ int wbox[4];          // written-node box of the accumulation tile
+    __shared__ int pbox[4];          // particle cell box of the chunk (min x, min y, -max x, -max y)
     const int tid = threadIdx.x, lane = tid & 63;
     const int base = blockIdx.x * kChunk;
     const int wx0 = ws->wx0, wy0 = ws->wy0, rx = ws->rx, ry = ws->ry;
@@ -412,6 +413,7 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     if (tid == 0) {
         sbox[0] = INT_MAX; sbox[1] = INT_MAX;
         wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN;
+        pbox[0] = INT_MAX; pbox[1] = INT_MAX; pbox[2] = INT_MAX; pbox[3] = INT_MAX;
     }
     for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
     __syncthreads();
@@ -613,12 +615,18 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             atomicMax(&wbox[2], tmaxx); atomicMax(&wbox[3], tmaxy);
         }
         if (bminx != INT_MAX) {
-            atomicMin(&ws->bb[0], bminx); atomicMin(&ws->bb[1], bminy);
-            atomicMin(&ws->bb[2], -bmaxx); atomicMin(&ws->bb[3], -bmaxy);
+            atomicMin(&pbox[0], bminx); atomicMin(&pbox[1], bminy);
+            atomicMin(&pbox[2], -bmaxx); atomicMin(&pbox[3], -bmaxy);
         }
         if (err) atomicOr(&ws->err, err);
     }
     __syncthreads();
+    // one global update of the window's particle box per workgroup: every
+    // workgroup targets the same four words, so per-wave atomics serialise
+    if (tid == 0 && pbox[0] != INT_MAX) {
+        atomicMin(&ws->bb[0], pbox[0]); atomicMin(&ws->bb[1], pbox[1]);
+        atomicMin(&ws->bb[2], pbox[2]); atomicMin(&ws->bb[3], pbox[3]);
+    }
     if (wbox[0] == INT_MAX) return;
     const int ww = wbox[2] - wbox[0] + 1, wh = wbox[3] - wbox[1] + 1;
     for (int q = tid; q < ww * wh; q += kB) {
@@ -1150,10 +1158,18 @@ __global__ void k_bbox_init(Parts<float> pf, Parts<double> pd, bool isf, WinStat
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = isf ? pf.n : pd.n;
-    if (k >= n) return;
-    const int gx = isf ? pf.gx[k] : pd.gx[k], gy = isf ? pf.gy[k] : pd.gy[k];
-    atomicMin(&ws->bb[0], gx); atomicMin(&ws->bb[1], gy);
-    atomicMin(&ws->bb[2], -gx); atomicMin(&ws->bb[3], -gy);
+    int b[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+    if (k < n) {
+        const int gx = isf ? pf.gx[k] : pd.gx[k], gy = isf ? pf.gy[k] : pd.gy[k];
+        b[0] = gx; b[1] = gy; b[2] = -gx; b[3] = -gy;
+    }
+    // wavefront minimum first: every thread targets the same four words
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = hu::wave_min(b[c]);
+    if ((threadIdx.x & 63) == 0 && b[0] != INT_MAX) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) atomicMin(&ws->bb[c], b[c]);
+    }
 }
 
 }  // namespace mw

@@ -59,6 +59,7 @@ def main():
     rhos, phis, cold = [], [], []
     for k in range(a.train + a.test):
         rho = sample(rng, H, W, h)
+        rho[cls != P.UNKNOWN] = 0.0             # grounded walls: identity rows give phi = 0
         phi, c, res = P.solve(h, h, cls, rho, tol=1e-10, maxit=300)
         rhos.append(rho.astype(np.float32))
         phis.append(phi.astype(np.float32))

@@ -66,6 +66,7 @@ def test_training_saves_vcycles(gpu):
     data = []
     for _ in range(96):
         rho = make_sample(rng, H, W, h)
+        rho[cls != P.UNKNOWN] = 0.0             # grounded walls: phi = 0 there
         phi, cyc, res = P.solve(h, h, cls, rho, tol=1e-12, maxit=100)
         data.append((rho, phi))
     # inputs scaled to peak 1, targets to unit RMS (the problem is linear)
@@ -81,7 +82,7 @@ def test_training_saves_vcycles(gpu):
             losses.append(u.train_step(rb, pb, lr=2e-4, scale=scale))
     first, last = np.mean(losses[:10]), np.mean(losses[-10:])
     print("loss %.3e -> %.3e" % (first, last))
-    assert last < 0.7 * first
+    assert last < 0.1 * first
     cold, warm = [], []
     for rho, _ in test:
         guess = u.infer(rho[None], scale=scale)[0].astype(np.float64) / pscale
