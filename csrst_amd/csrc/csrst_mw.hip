@@ -386,10 +386,61 @@ __device__ __forceinline__ void seg_s2_sep(pm::SegCurrent<T> &o, int comp, T tau
     }
 }
 
-constexpr int kPPT = 8;              // particles per thread in one workgroup chunk
+constexpr int kRun = 4;              // consecutive particles per vector load (16 B of FP32, 32 B of FP64)
+constexpr int kPPT = 8;              // particles per thread: two runs of kRun consecutive sorted particles
 constexpr int kChunk = kB * kPPT;    // 2048 particles per workgroup
 constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
 
+// Particle arrays start at multiples of 64 elements (Parts stride), so a run
+// of kRun values at a multiple of kRun is one aligned vector access.
+template <class T>
+struct alignas(16) Run {
+    T v[kRun];
+};
+template <class T>
+__device__ __forceinline__ void load_run(const T *a, int k0, int n, T (&o)[kRun])
+{
+    if (k0 + kRun <= n) {
+        const Run<T> r = *reinterpret_cast<const Run<T> *>(a + k0);
+#pragma unroll
+        for (int u = 0; u < kRun; ++u) o[u] = r.v[u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < kRun; ++u) o[u] = k0 + u < n ? a[k0 + u] : T(0);
+    }
+}
+template <class T>
+__device__ __forceinline__ void store_run(T *a, int k0, int n, const T (&o)[kRun])
+{
+    if (k0 + kRun <= n) {
+        Run<T> r;
+#pragma unroll
+        for (int u = 0; u < kRun; ++u) r.v[u] = o[u];
+        *reinterpret_cast<Run<T> *>(a + k0) = r;
+    } else {
+#pragma unroll
+        for (int u = 0; u < kRun; ++u)
+            if (k0 + u < n) a[k0 + u] = o[u];
+    }
+}
+// shift a run down by one register and append v (keeps every index static)
+template <class T>
+__device__ __forceinline__ void rotate_in(T (&o)[kRun], T v)
+{
+#pragma unroll
+    for (int u = 0; u < kRun - 1; ++u) o[u] = o[u + 1];
+    o[kRun - 1] = v;
+}
+
+// Push of one workgroup chunk (kChunk sorted particles): thread t owns
+// particles base + kPPT t .. + kPPT - 1, so its consecutive particles share a
+// cell and their segment currents share stencil anchors.  Currents collect in
+// two register slots per lane (one anchor each); a segment whose anchor
+// matches neither (a run crossing a cell boundary) is added by its lane into
+// the chunk's LDS tile, and the slots at the end are reduced across the
+// wavefront once per distinct anchor (DPP sums).  The tile is flushed with one
+// global atomic per touched node.  Loops stay rolled where unrolling would
+// not pay: the kernel has to fit the instruction cache.
 template <class T, int ORDER, int COMP>
 __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
@@ -399,15 +450,22 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     __shared__ int pbox[4];          // particle cell box of the chunk (min x, min y, -max x, -max y)
     const int tid = threadIdx.x, lane = tid & 63;
     const int base = blockIdx.x * kChunk;
+    const int kt = base + tid * kPPT;           // this thread's first particle
     const int wx0 = ws->wx0, wy0 = ws->wy0, rx = ws->rx, ry = ws->ry;
     const vec2<T> p0 = pm::mk((T)ws->p0[0], (T)ws->p0[1]);
     const int W = f.W, H = f.H;
     T *J = COMP == 0 ? f.jx : f.jy;
     // ---- tile anchor: minimum old cell of the chunk
     int mnx = INT_MAX, mny = INT_MAX;
-    for (int it = 0; it < kPPT; ++it) {
-        const int k = base + it * kB + tid;
-        if (k < p.n) { mnx = min(mnx, p.gx[k]); mny = min(mny, p.gy[k]); }
+#pragma unroll
+    for (int r = 0; r < kPPT / kRun; ++r) {
+        int ax4[kRun], ay4[kRun];
+        const int k0 = kt + r * kRun;
+        load_run(p.gx, k0, p.n, ax4);
+        load_run(p.gy, k0, p.n, ay4);
+#pragma unroll
+        for (int u = 0; u < kRun; ++u)
+            if (k0 + u < p.n) { mnx = min(mnx, ax4[u]); mny = min(mny, ay4[u]); }
     }
     mnx = hu::wave_min(mnx); mny = hu::wave_min(mny);
     if (tid == 0) {
@@ -428,20 +486,42 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     const vec2<T> odt0 = pm::mk(c.odt_y, c.odt_x);   // S0: (1/(hy dt), 1/(hx dt)), ZerothOrderShapeFunction.cpp:224
     const vec2<T> qodt = pm::mk(c.qodt_x, c.qodt_y);
     constexpr int NN = NX * NY;
-    // per-lane add of one stencil into the LDS tile (global atomics off-tile)
+    // one node value from one lane: LDS tile, else a global atomic
+    auto node_add = [&](int gxn, int gyn, T v) {
+        const int li = gxn - ax, lj = gyn - ay;
+        if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
+            atomicAdd(&acc[lj * kTW + li], v);
+            tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+        } else {
+            const int i = gxn - wx0, j = gyn - wy0;
+            if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
+            else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
+        }
+    };
+    // per-lane add of one stencil (LDS tile, global atomics off the tile):
+    // for the rare segment whose anchor matches neither register slot
     auto lane_add = [&](int I0, int J0, const T *w) {
 #pragma unroll
-        for (int t = 0; t < NN; ++t) {
-            const int gxn = I0 + (t % NX), gyn = J0 + (t / NX);
-            const int li = gxn - ax, lj = gyn - ay;
-            if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
-                atomicAdd(&acc[lj * kTW + li], w[t]);
-                tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
-            } else {
-                const int i = gxn - wx0, j = gyn - wy0;
-                if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { err |= 1; continue; }
-                atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], w[t]);
+        for (int t = 0; t < NN; ++t) node_add(I0 + (t % NX), J0 + (t / NX), w[t]);
+    };
+    // Wavefront-cooperative add of the lanes' stencils (lanes with `mine`
+    // set), one pass per distinct anchor: the NN node sums are formed with DPP
+    // and handed to lanes 0 .. NN-1, which add them in one step.  Called in
+    // wave-uniform control flow (every lane active).
+    auto coop_add = [&](bool mine0, int sI, int sJ, const T *w) {
+        unsigned long long rem = __ballot(mine0);
+        while (rem) {
+            const int lead = __ffsll((long long)rem) - 1;
+            const int lI = __builtin_amdgcn_readlane(sI, lead), lJ = __builtin_amdgcn_readlane(sJ, lead);
+            const bool mine = mine0 && sI == lI && sJ == lJ;
+            rem &= ~__ballot(mine);
+            T v = T(0);
+#pragma unroll
+            for (int t = 0; t < NN; ++t) {
+                const T st = wave_sum(mine ? w[t] : T(0));
+                v = lane == t ? st : v;
             }
+            if (lane < NN) node_add(lI + (lane % NX), lJ + (lane / NX), v);
         }
     };
     // two register accumulation slots per lane
@@ -450,158 +530,153 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     T accA[NN], accB[NN];
 #pragma unroll
     for (int t = 0; t < NN; ++t) { accA[t] = T(0); accB[t] = T(0); }
-    for (int it = 0; it < kPPT; ++it) {
-        const int k = base + it * kB + tid;
-        const bool live = k < p.n;
-        int gx0 = 0, gy0 = 0, gx = 0, gy = 0;
-        T fx0 = 0, fy0 = 0, fx = 0, fy = 0;
-        vec2<T> dP = pm::mk(T(0), T(0)), P = dP;
-        T g2 = 0;
-        if (live) {
-            gx0 = p.gx[k]; gy0 = p.gy[k];
-            fx0 = p.fx[k]; fy0 = p.fy[k];
-            dP = pm::mk(p.px[k], p.py[k]);
-            P = pm::mk(p0.x + dP.x, p0.y + dP.y);
-            // half drift (PartBunch.cpp:228-232)
-            const T gi = pm::inv_gamma(P);
-            gx = gx0; gy = gy0; fx = fx0; fy = fy0;
-            split_add(gx, fx, c.hx * P.x * gi);
-            split_add(gy, fy, c.hy * P.y * gi);
-            // gather at R1 (positions relative to the particle's cell corner);
-            // ring offsets of the stencil rows/columns computed once per field
-            const vec2<T> pp = pm::mk(fx, fy);
-            constexpr int N = ORDER + 2;
-            // Fast path: the N x N block lies inside the window and does not
-            // straddle the ring seam, so it is N rows of N contiguous values.
-            auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
-                const int i0 = gx + (int)st.I0 - wx0, j0 = gy + (int)st.J0 - wy0;
-                int px0 = i0 + G + rx, py0 = j0 + G + ry;
-                px0 = px0 >= f.Wp ? px0 - f.Wp : px0;
-                py0 = py0 >= f.Hp ? py0 - f.Hp : py0;
-                const bool fast = i0 >= -G && i0 + N - 1 <= W - 1 + G && j0 >= -G && j0 + N - 1 <= H - 1 + G &&
-                                  px0 >= 0 && py0 >= 0 && px0 + N - 1 < f.Wp && py0 + N - 1 < f.Hp;
-                if (fast) {
-                    const T *b0 = plane + (size_t)py0 * f.Wp + px0;
-                    return pm::gather_n<N>(st, false, [&](int a, int b) { return b0[(size_t)b * f.Wp + a]; });
-                }
-                int col[N];
-                size_t rowb[N];
+#pragma unroll 1
+    for (int r = 0; r < kPPT / kRun; ++r) {
+        const int k0 = kt + r * kRun;
+        int GX[kRun], GY[kRun];
+        T FX[kRun], FY[kRun], PX[kRun], PY[kRun];
+        load_run(p.gx, k0, p.n, GX);
+        load_run(p.gy, k0, p.n, GY);
+        load_run(p.fx, k0, p.n, FX);
+        load_run(p.fy, k0, p.n, FY);
+        load_run(p.px, k0, p.n, PX);
+        load_run(p.py, k0, p.n, PY);
+#pragma unroll 1
+        for (int u = 0; u < kRun; ++u) {
+            const bool live = k0 + u < p.n;
+            const int gx0 = GX[0], gy0 = GY[0];
+            const T fx0 = FX[0], fy0 = FY[0];
+            vec2<T> dP = pm::mk(PX[0], PY[0]), P = dP;
+            int gx = gx0, gy = gy0;
+            T fx = fx0, fy = fy0;
+            T g2 = 0;
+            if (live) {
+                P = pm::mk(p0.x + dP.x, p0.y + dP.y);
+                // half drift (PartBunch.cpp:228-232)
+                const T gi = pm::inv_gamma(P);
+                split_add(gx, fx, c.hx * P.x * gi);
+                split_add(gy, fy, c.hy * P.y * gi);
+                // gather at R1 (positions relative to the particle's cell corner);
+                // ring offsets of the stencil rows/columns computed once per field
+                const vec2<T> pp = pm::mk(fx, fy);
+                constexpr int N = ORDER + 2;
+                // Fast path: the N x N block lies inside the window and does not
+                // straddle the ring seam, so it is N rows of N contiguous values.
+                auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
+                    const int i0 = gx + (int)st.I0 - wx0, j0 = gy + (int)st.J0 - wy0;
+                    int px0 = i0 + G + rx, py0 = j0 + G + ry;
+                    px0 = px0 >= f.Wp ? px0 - f.Wp : px0;
+                    py0 = py0 >= f.Hp ? py0 - f.Hp : py0;
+                    const bool fast = i0 >= -G && i0 + N - 1 <= W - 1 + G && j0 >= -G && j0 + N - 1 <= H - 1 + G &&
+                                      px0 >= 0 && py0 >= 0 && px0 + N - 1 < f.Wp && py0 + N - 1 < f.Hp;
+                    if (fast) {
+                        const T *b0 = plane + (size_t)py0 * f.Wp + px0;
+                        return pm::gather_n<N>(st, false, [&](int a, int b) { return b0[(size_t)b * f.Wp + a]; });
+                    }
+                    int col[N];
+                    size_t rowb[N];
 #pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    const int i = i0 + a, j = j0 + a;
-                    if (i < -G || i > W - 1 + G || j < -G || j > H - 1 + G) err |= 1;
-                    col[a] = ringx(min(max(i, -G), W - 1 + G), rx, f.Wp);
-                    rowb[a] = (size_t)ringy(min(max(j, -G), H - 1 + G), ry, f.Hp) * f.Wp;
-                }
-                return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
-            };
-            const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
-            const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
-            const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
-            const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
-            const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
-            const T bext = (T)geom::external_bz(g.bend, X, Y);
-            dP = kick_rotate_split(p0, dP, ex, ey, hf, bext, c.fact1, c.fact2);
-            P = pm::mk(p0.x + dP.x, p0.y + dP.y);
-            g2 = pm::inv_gamma(P);
-            split_add(gx, fx, c.hx * P.x * g2);
-            split_add(gy, fy, c.hy * P.y * g2);
-        }
-        // ---- deposit of R0 -> R2, cell units relative to the old cell corner
-        const vec2<T> o = pm::mk(fx0, fy0);
-        const vec2<T> e = pm::mk((T)(gx - gx0) + fx, (T)(gy - gy0) + fy);
-        Split<T> xs;
-        xs.n = 1;
-        if (live && !split_path(xs, o, e, ORDER == 1)) err |= 2;
-        const vec2<T> delta = pm::mk(e.x - o.x, e.y - o.y);
-        const int nseg = live ? xs.n - 1 : 0;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            if (s >= nseg) continue;
-            pm::SegCurrent<T> sc;
-            {
-                // slot selects instead of xs.s[s] keep the list in registers
-                const vec2<T> xi = sel(s == 0, xs.s[0], sel(s == 1, xs.s[1], xs.s[2]));
-                const vec2<T> xf = sel(s == 0, xs.s[1], sel(s == 1, xs.s[2], xs.s[3]));
-                const T fi = s == 0 ? xs.f[0] : (s == 1 ? xs.f[1] : xs.f[2]);
-                const T ff = s == 0 ? xs.f[1] : (s == 1 ? xs.f[2] : xs.f[3]);
-                if (ORDER == 0) {
-                    pm::seg_s0(sc, COMP, xi, xf, c.q, odt0);
-                } else {
-                    const T tau = ff - fi;
-                    const vec2<T> cc = pm::mk(T(0.5) * (xi.x + xf.x), T(0.5) * (xi.y + xf.y));
-                    const int I = ORDER == 1 ? (int)pm::floor_(cc.x + T(0.5)) : (int)pm::floor_(cc.x);
-                    const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
-                    const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
-                    const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
-                    if (ORDER == 1) seg_s1_sep(sc, COMP, tau, a, b, qodt, delta);
-                    else seg_s2_sep(sc, COMP, tau, a, b, qodt, delta);
-                    sc.I0 = I - 1; sc.J0 = Jn - 1;
-                }
-                sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
+                    for (int a = 0; a < N; ++a) {
+                        const int i = i0 + a, j = j0 + a;
+                        if (i < -G || i > W - 1 + G || j < -G || j > H - 1 + G) err |= 1;
+                        col[a] = ringx(min(max(i, -G), W - 1 + G), rx, f.Wp);
+                        rowb[a] = (size_t)ringy(min(max(j, -G), H - 1 + G), ry, f.Hp) * f.Wp;
+                    }
+                    return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
+                };
+                const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
+                const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
+                const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
+                const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
+                const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
+                const T bext = (T)geom::external_bz(g.bend, X, Y);
+                dP = kick_rotate_split(p0, dP, ex, ey, hf, bext, c.fact1, c.fact2);
+                P = pm::mk(p0.x + dP.x, p0.y + dP.y);
+                g2 = pm::inv_gamma(P);
+                split_add(gx, fx, c.hx * P.x * g2);
+                split_add(gy, fy, c.hy * P.y * g2);
             }
-            // register accumulation: consecutive particles of a thread sit in
-            // the same cell of the sorted order, so their segments share anchors
-            if (hasA && sc.I0 == aI && sc.J0 == aJ) {
-#pragma unroll
-                for (int t = 0; t < NN; ++t) accA[t] += sc.w[t];
-            } else if (hasB && sc.I0 == bI && sc.J0 == bJ) {
-#pragma unroll
-                for (int t = 0; t < NN; ++t) accB[t] += sc.w[t];
-            } else if (!hasA) {
-                hasA = true; aI = sc.I0; aJ = sc.J0;
-#pragma unroll
-                for (int t = 0; t < NN; ++t) accA[t] = sc.w[t];
-            } else if (!hasB) {
-                hasB = true; bI = sc.I0; bJ = sc.J0;
-#pragma unroll
-                for (int t = 0; t < NN; ++t) accB[t] = sc.w[t];
-            } else {
-                lane_add(sc.I0, sc.J0, sc.w);
-            }
-        }
-        // ---- drift back by half a step (TETMSolver.hh:210, 265) and store
-        if (live) {
-            bminx = min(bminx, min(gx0, gx)); bminy = min(bminy, min(gy0, gy));
-            bmaxx = max(bmaxx, max(gx0, gx)); bmaxy = max(bmaxy, max(gy0, gy));
-            split_add(gx, fx, -c.hx * P.x * g2);
-            split_add(gy, fy, -c.hy * P.y * g2);
-            p.gx[k] = gx; p.gy[k] = gy;
-            p.fx[k] = fx; p.fy[k] = fy;
-            p.px[k] = dP.x; p.py[k] = dP.y;
-        }
-    }
-    // ---- flush the register slots: one DPP wavefront sum per node when every
-    // lane holding the slot shares its anchor, per-lane LDS adds otherwise
-    auto flush_slot = [&](bool has, int sI, int sJ, const T *a) {
-        const unsigned long long hm = __ballot(has);
-        if (hm == 0) return;
-        const int lead = __ffsll((long long)hm) - 1;
-        const int lI = __shfl(sI, lead, 64), lJ = __shfl(sJ, lead, 64);
-        const bool uniform = __ballot(has && (sI != lI || sJ != lJ)) == 0;
-        if (uniform) {
-#pragma unroll
-            for (int t = 0; t < NN; ++t) {
-                const T v = wave_sum(has ? a[t] : T(0));
-                if (lane == lead) {
-                    const int gxn = lI + (t % NX), gyn = lJ + (t / NX);
-                    const int li = gxn - ax, lj = gyn - ay;
-                    if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
-                        atomicAdd(&acc[lj * kTW + li], v);
-                        tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
+            // ---- deposit of R0 -> R2, cell units relative to the old cell corner
+            const vec2<T> o = pm::mk(fx0, fy0);
+            const vec2<T> e = pm::mk((T)(gx - gx0) + fx, (T)(gy - gy0) + fy);
+            Split<T> xs;
+            const bool oneCross = split_path(xs, o, e, ORDER == 1);   // dead lanes: o = e = 0
+            if (live && !oneCross) err |= 2;
+            const vec2<T> delta = pm::mk(e.x - o.x, e.y - o.y);
+            const int nseg = live ? xs.n - 1 : 0;
+#pragma unroll 1
+            for (int s = 0; s < 3; ++s) {
+                const bool act = s < nseg;
+                if (!__any(act)) break;
+                pm::SegCurrent<T> sc;
+                {
+                    // slot selects instead of xs.s[s] keep the list in registers
+                    const vec2<T> xi = sel(s == 0, xs.s[0], sel(s == 1, xs.s[1], xs.s[2]));
+                    const vec2<T> xf = sel(s == 0, xs.s[1], sel(s == 1, xs.s[2], xs.s[3]));
+                    const T fi = s == 0 ? xs.f[0] : (s == 1 ? xs.f[1] : xs.f[2]);
+                    const T ff = s == 0 ? xs.f[1] : (s == 1 ? xs.f[2] : xs.f[3]);
+                    if (ORDER == 0) {
+                        pm::seg_s0(sc, COMP, xi, xf, c.q, odt0);
                     } else {
-                        const int i = gxn - wx0, j = gyn - wy0;
-                        if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
-                        else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
+                        const T tau = ff - fi;
+                        const vec2<T> cc = pm::mk(T(0.5) * (xi.x + xf.x), T(0.5) * (xi.y + xf.y));
+                        const int I = ORDER == 1 ? (int)pm::floor_(cc.x + T(0.5)) : (int)pm::floor_(cc.x);
+                        const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
+                        const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
+                        const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
+                        if (ORDER == 1) seg_s1_sep(sc, COMP, tau, a, b, qodt, delta);
+                        else seg_s2_sep(sc, COMP, tau, a, b, qodt, delta);
+                        sc.I0 = I - 1; sc.J0 = Jn - 1;
+                    }
+                    sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
+                }
+                // register accumulation by anchor
+                if (act) {
+                    if (hasA && sc.I0 == aI && sc.J0 == aJ) {
+#pragma unroll
+                        for (int t = 0; t < NN; ++t) accA[t] += sc.w[t];
+                    } else if (hasB && sc.I0 == bI && sc.J0 == bJ) {
+#pragma unroll
+                        for (int t = 0; t < NN; ++t) accB[t] += sc.w[t];
+                    } else if (!hasA) {
+                        hasA = true; aI = sc.I0; aJ = sc.J0;
+#pragma unroll
+                        for (int t = 0; t < NN; ++t) accA[t] = sc.w[t];
+                    } else if (!hasB) {
+                        hasB = true; bI = sc.I0; bJ = sc.J0;
+#pragma unroll
+                        for (int t = 0; t < NN; ++t) accB[t] = sc.w[t];
+                    } else {
+                        lane_add(sc.I0, sc.J0, sc.w);
                     }
                 }
             }
-        } else if (has) {
-            lane_add(sI, sJ, a);
+            // ---- drift back by half a step (TETMSolver.hh:210, 265)
+            if (live) {
+                bminx = min(bminx, min(gx0, gx)); bminy = min(bminy, min(gy0, gy));
+                bmaxx = max(bmaxx, max(gx0, gx)); bmaxy = max(bmaxy, max(gy0, gy));
+                split_add(gx, fx, -c.hx * P.x * g2);
+                split_add(gy, fy, -c.hy * P.y * g2);
+            }
+            rotate_in(GX, gx); rotate_in(GY, gy);
+            rotate_in(FX, fx); rotate_in(FY, fy);
+            rotate_in(PX, dP.x); rotate_in(PY, dP.y);
         }
-    };
-    flush_slot(hasA, aI, aJ, accA);
-    flush_slot(hasB, bI, bJ, accB);
+        store_run(p.gx, k0, p.n, GX);
+        store_run(p.gy, k0, p.n, GY);
+        store_run(p.fx, k0, p.n, FX);
+        store_run(p.fy, k0, p.n, FY);
+        store_run(p.px, k0, p.n, PX);
+        store_run(p.py, k0, p.n, PY);
+    }
+    // ---- flush the register slots
+#pragma unroll 1
+    for (int slot = 0; slot < 2; ++slot) {
+        T w[NN];
+#pragma unroll
+        for (int t = 0; t < NN; ++t) w[t] = slot == 0 ? accA[t] : accB[t];
+        coop_add(slot == 0 ? hasA : hasB, slot == 0 ? aI : bI, slot == 0 ? aJ : bJ, w);
+    }
     // ---- written-node box of the tile, then flush the tile with one global
     // atomic per touched node
     tminx = hu::wave_min(tminx); tminy = hu::wave_min(tminy);
@@ -1227,12 +1302,16 @@ struct csrst_mw {
         f.pml = npml ? pb + 3 * plane : nullptr;
         return f;
     }
+    // elements per particle array: a multiple of 64, so every array starts
+    // 256-byte aligned and runs of consecutive particles load as vectors
+    size_t stride() const { return ((size_t)n + 63) & ~(size_t)63; }
+    size_t part_bytes() const { return stride() * (8 + 4 * (f32 ? 4 : 8)); }
     template <class T>
     mw::Parts<T> parts(DBuf<unsigned char> &buf)
     {
         mw::Parts<T> pp;
         unsigned char *b = buf.p;
-        const size_t nn = (size_t)n;
+        const size_t nn = stride();
         pp.gx = reinterpret_cast<int *>(b);
         pp.gy = reinterpret_cast<int *>(b + 4 * nn);
         T *r = reinterpret_cast<T *>(b + 8 * nn);
@@ -1555,7 +1634,8 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
         std::vector<int> gx(n), gy(n);
         const bool f32 = m->f32;
         const size_t rs = f32 ? 4 : 8;
-        std::vector<unsigned char> host((size_t)n * (8 + 4 * rs));
+        const size_t ns = m->stride();
+        std::vector<unsigned char> host(m->part_bytes(), 0);
         int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
         long double sp[2] = {0, 0};
         for (int k = 0; k < n; ++k) { sp[0] += P[2 * k]; sp[1] += P[2 * k + 1]; }
@@ -1569,10 +1649,10 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
             mnx = std::min(mnx, gx[k]); mxx = std::max(mxx, gx[k]);
             mny = std::min(mny, gy[k]); mxy = std::max(mxy, gy[k]);
             memcpy(&host[4 * (size_t)k], &gx[k], 4);
-            memcpy(&host[4 * (size_t)n + 4 * (size_t)k], &gy[k], 4);
+            memcpy(&host[4 * ns + 4 * (size_t)k], &gy[k], 4);
             const double v[4] = {ux - fx, uy - fy, P[2 * k] - p0[0], P[2 * k + 1] - p0[1]};
             for (int c = 0; c < 4; ++c) {
-                unsigned char *dst = &host[8 * (size_t)n + ((size_t)c * n + k) * rs];
+                unsigned char *dst = &host[8 * ns + ((size_t)c * ns + k) * rs];
                 if (f32) {
                     float fv = (float)v[c];
                     if (c < 2 && fv >= 1.0f) fv = nextafterf(1.0f, 0.0f);
@@ -1781,18 +1861,18 @@ int csrst_mw_download_particles(csrst_mw *m, double *R, double *P, int n)
     try {
         if (!m || n != m->n) throw Error(CSRST_EINVAL, "particle count mismatch");
         HU_CHECK(hipStreamSynchronize(m->s));
-        const size_t rs = m->f32 ? 4 : 8;
-        std::vector<unsigned char> host((size_t)n * (8 + 4 * rs));
+        const size_t rs = m->f32 ? 4 : 8, ns = m->stride();
+        std::vector<unsigned char> host(m->part_bytes());
         HU_CHECK(hipMemcpy(host.data(), m->partBuf.p, host.size(), hipMemcpyDeviceToHost));
         mw::WinState hs;
         HU_CHECK(hipMemcpy(&hs, m->ws.p, sizeof(hs), hipMemcpyDeviceToHost));
         for (int k = 0; k < n; ++k) {
             int gx, gy;
             memcpy(&gx, &host[4 * (size_t)k], 4);
-            memcpy(&gy, &host[4 * (size_t)n + 4 * (size_t)k], 4);
+            memcpy(&gy, &host[4 * ns + 4 * (size_t)k], 4);
             double v[4];
             for (int c = 0; c < 4; ++c) {
-                const unsigned char *src = &host[8 * (size_t)n + ((size_t)c * n + k) * rs];
+                const unsigned char *src = &host[8 * ns + ((size_t)c * ns + k) * rs];
                 if (m->f32) { float fv; memcpy(&fv, src, 4); v[c] = fv; }
                 else memcpy(&v[c], src, 8);
             }
