@@ -50,11 +50,16 @@ constexpr int kB = 256;
 constexpr int G = 1;  // guard layer
 
 // Device-resident window state (changes every step without host round trips).
+constexpr int kBBSlots = 64;
+
 struct WinState {
     int wx0, wy0;        // global cell index of logical cell (0, 0)
     int rx, ry;          // ring offsets
     int mi, mj;          // last move
-    int bb[4];           // particle bounding box in global cells: min x, min y, -max x, -max y (atomicMin)
+    int bb[4];           // particle bounding box in global cells: min x, min y, -max x, -max y
+    // per-slot boxes (atomicMin), one 128-byte line each: workgroups spread
+    // their updates over kBBSlots addresses, k_move reduces and clears them
+    int bbs[kBBSlots][32];
     int jbox[2][4];      // current footprint per component (global nodes): lo x, lo y, -hi x, -hi y
     int err;             // 1: stencil left the window, 2: too many crossings, 4: move out of range
     int steps;
@@ -158,6 +163,11 @@ __device__ __forceinline__ int wave_or(int v)
 // P = p0 + dP: the increments are formed directly so that FP32 keeps the
 // resolution of dP (|dP| << |P|).  Algebraically identical to
 // P' = ((1+a^2) s + a s_perp)/(1+a^2) + kick, s = um + a (um_y, -um_x).
+// Quotient of the split-precision form: FP32 uses the hardware reciprocal
+// (1 ulp) instead of the IEEE division sequence; FP64 divides exactly.
+__device__ __forceinline__ float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double quot(double a, double b) { return a / b; }
+
 template <class T>
 __device__ __forceinline__ vec2<T> kick_rotate_split(vec2<T> p0, vec2<T> dP, T ex, T ey, T hf, T bext, T fact1, T fact2)
 {
@@ -166,13 +176,15 @@ __device__ __forceinline__ vec2<T> kick_rotate_split(vec2<T> p0, vec2<T> dP, T e
     const vec2<T> um = pm::mk(p0.x + (dP.x + kick.x), p0.y + (dP.y + kick.y));
     const T a = fact2 * B * (T)pm::q_rsqrt((float)(T(1.0) + (um.x * um.x + um.y * um.y)));
     const T a2 = a * a;
-    const T r = T(1.0) / (T(1.0) + a2);
+    const T r = quot(T(1.0), T(1.0) + a2);
     const T c1 = a * (T(1.0) + r);
     const T dx = c1 * um.y - a2 * r * um.x;
     const T dy = -c1 * um.x - a2 * r * um.y;
     return pm::mk(dP.x + ((kick.x + kick.x) + dx), dP.y + ((kick.y + kick.y) + dy));
 }
 
+// f stays in [0, 1): f - floor(f) rounds to 1 for a tiny negative f, which
+// is renormalised to the next cell (the same position)
 template <class T>
 __device__ __forceinline__ void split_add(int &g, T &f, T d)
 {
@@ -180,6 +192,9 @@ __device__ __forceinline__ void split_add(int &g, T &f, T d)
     const T fl = pm::floor_(f);
     g += (int)fl;
     f = f - fl;
+    const bool one = f >= T(1);
+    g += one ? 1 : 0;
+    f = one ? f - T(1) : f;
 }
 
 // Wave-level (64 lanes) sum with DPP row shifts and row broadcasts: lane 63
@@ -266,13 +281,13 @@ __device__ __forceinline__ bool split_path(Split<T> &c, vec2<T> o, vec2<T> e, bo
     T sax = o.x, say = o.y, sbx = o.x, sby = o.y;
     if (dI != 0) {
         const T xb = (T)(dI > 0 ? If : Ii) + sh;
-        fa = (xb - o.x) / D.x;
+        fa = quot(xb - o.x, D.x);
         sax = xb;
         say = o.y + fa * D.y;
     }
     if (dJ != 0) {
         const T yb = (T)(dJ > 0 ? Jf : Ji) + sh;
-        fb = (yb - o.y) / D.y;
+        fb = quot(yb - o.y, D.y);
         sbx = o.x + fb * D.x;
         sby = yb;
     }
@@ -387,8 +402,8 @@ __device__ __forceinline__ void seg_s2_sep(pm::SegCurrent<T> &o, int comp, T tau
 }
 
 constexpr int kRun = 4;              // consecutive particles per vector load (16 B of FP32, 32 B of FP64)
-constexpr int kPPT = 8;              // particles per thread: two runs of kRun consecutive sorted particles
-constexpr int kChunk = kB * kPPT;    // 2048 particles per workgroup
+constexpr int kPPT = 4;              // particles per thread: one run of kRun consecutive sorted particles
+constexpr int kChunk = kB * kPPT;    // 1024 particles per workgroup
 constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
 
 // Particle arrays start at multiples of 64 elements (Parts stride), so a run
@@ -445,7 +460,6 @@ template <class T, int ORDER, int COMP>
 __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
     __shared__ T acc[kTH * kTW];
-    __shared__ int sbox[2];          // chunk's minimum old cell
     __shared__ int wbox[4];          // written-node box of the accumulation tile
     __shared__ int pbox[4];          // particle cell box of the chunk (min x, min y, -max x, -max y)
     const int tid = threadIdx.x, lane = tid & 63;
@@ -455,29 +469,17 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     const vec2<T> p0 = pm::mk((T)ws->p0[0], (T)ws->p0[1]);
     const int W = f.W, H = f.H;
     T *J = COMP == 0 ? f.jx : f.jy;
-    // ---- tile anchor: minimum old cell of the chunk
-    int mnx = INT_MAX, mny = INT_MAX;
-#pragma unroll
-    for (int r = 0; r < kPPT / kRun; ++r) {
-        int ax4[kRun], ay4[kRun];
-        const int k0 = kt + r * kRun;
-        load_run(p.gx, k0, p.n, ax4);
-        load_run(p.gy, k0, p.n, ay4);
-#pragma unroll
-        for (int u = 0; u < kRun; ++u)
-            if (k0 + u < p.n) { mnx = min(mnx, ax4[u]); mny = min(mny, ay4[u]); }
-    }
-    mnx = hu::wave_min(mnx); mny = hu::wave_min(mny);
+    // ---- tile anchor from the chunk's first particle: in the tile-major sort
+    // order the chunk continues right and down from it, so the tile covers
+    // 20 nodes left and 6 up of that cell (stencil reach included); nodes of a
+    // sparse chunk that fall outside go to global atomics
+    const int ax = p.gx[base] - 20, ay = p.gy[base] - 6;   // LDS node (0,0) = global node (ax, ay)
     if (tid == 0) {
-        sbox[0] = INT_MAX; sbox[1] = INT_MAX;
         wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN;
         pbox[0] = INT_MAX; pbox[1] = INT_MAX; pbox[2] = INT_MAX; pbox[3] = INT_MAX;
     }
     for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
     __syncthreads();
-    if (lane == 0) { atomicMin(&sbox[0], mnx); atomicMin(&sbox[1], mny); }
-    __syncthreads();
-    const int ax = sbox[0] - 2, ay = sbox[1] - 2;   // LDS node (0,0) = global node (ax, ay)
     int err = 0;
     int bminx = INT_MAX, bminy = INT_MAX, bmaxx = INT_MIN, bmaxy = INT_MIN;
     int tminx = INT_MAX, tminy = INT_MAX, tmaxx = INT_MIN, tmaxy = INT_MIN;
@@ -556,33 +558,37 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
                 const T gi = pm::inv_gamma(P);
                 split_add(gx, fx, c.hx * P.x * gi);
                 split_add(gy, fy, c.hy * P.y * gi);
-                // gather at R1 (positions relative to the particle's cell corner);
-                // ring offsets of the stencil rows/columns computed once per field
+                // gather at R1 (positions relative to the particle's cell corner).
+                // With fx, fy in [0, 1) every stencil of the three fields starts at
+                // node offset L or L + 1 from the cell and spans N nodes, so the
+                // ring-wrapped byte offsets of the U = N + 1 union columns and rows
+                // are formed once and each load is plane + row + column (32-bit
+                // offsets from the uniform plane base).
                 const vec2<T> pp = pm::mk(fx, fy);
-                constexpr int N = ORDER + 2;
-                // Fast path: the N x N block lies inside the window and does not
-                // straddle the ring seam, so it is N rows of N contiguous values.
-                auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
-                    const int i0 = gx + (int)st.I0 - wx0, j0 = gy + (int)st.J0 - wy0;
-                    int px0 = i0 + G + rx, py0 = j0 + G + ry;
-                    px0 = px0 >= f.Wp ? px0 - f.Wp : px0;
-                    py0 = py0 >= f.Hp ? py0 - f.Hp : py0;
-                    const bool fast = i0 >= -G && i0 + N - 1 <= W - 1 + G && j0 >= -G && j0 + N - 1 <= H - 1 + G &&
-                                      px0 >= 0 && py0 >= 0 && px0 + N - 1 < f.Wp && py0 + N - 1 < f.Hp;
-                    if (fast) {
-                        const T *b0 = plane + (size_t)py0 * f.Wp + px0;
-                        return pm::gather_n<N>(st, false, [&](int a, int b) { return b0[(size_t)b * f.Wp + a]; });
-                    }
-                    int col[N];
-                    size_t rowb[N];
+                constexpr int N = ORDER + 2, L = ORDER == 2 ? -2 : -1, U = N + 1;
+                uint32_t C[U], R[U];
+                {
+                    const int i0 = gx + L - wx0, j0 = gy + L - wy0;
+                    if (i0 < -G || i0 + U - 1 > W - 1 + G || j0 < -G || j0 + U - 1 > H - 1 + G) err |= 1;
 #pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        const int i = i0 + a, j = j0 + a;
-                        if (i < -G || i > W - 1 + G || j < -G || j > H - 1 + G) err |= 1;
-                        col[a] = ringx(min(max(i, -G), W - 1 + G), rx, f.Wp);
-                        rowb[a] = (size_t)ringy(min(max(j, -G), H - 1 + G), ry, f.Hp) * f.Wp;
+                    for (int a = 0; a < U; ++a) {
+                        int px = min(max(i0 + a, -G), W - 1 + G) + G + rx;
+                        px = px >= f.Wp ? px - f.Wp : px;
+                        C[a] = (uint32_t)px * (uint32_t)sizeof(T);
+                        int py = min(max(j0 + a, -G), H - 1 + G) + G + ry;
+                        py = py >= f.Hp ? py - f.Hp : py;
+                        R[a] = (uint32_t)py * (uint32_t)(f.Wp * sizeof(T));
                     }
-                    return pm::gather_n<N>(st, false, [&](int a, int b) { return plane[rowb[b] + col[a]]; });
+                }
+                auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
+                    const bool sx = st.I0 != L, sy = st.J0 != L;
+                    uint32_t cc[N], rr[N];
+#pragma unroll
+                    for (int a = 0; a < N; ++a) { cc[a] = sx ? C[a + 1] : C[a]; rr[a] = sy ? R[a + 1] : R[a]; }
+                    const char *pb = reinterpret_cast<const char *>(plane);
+                    return pm::gather_n<N>(st, false, [&](int a, int b) {
+                        return *reinterpret_cast<const T *>(pb + (rr[b] + cc[a]));
+                    });
                 };
                 const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
                 const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
@@ -696,11 +702,12 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         if (err) atomicOr(&ws->err, err);
     }
     __syncthreads();
-    // one global update of the window's particle box per workgroup: every
-    // workgroup targets the same four words, so per-wave atomics serialise
+    // one update of the window's particle box per workgroup, into one of
+    // kBBSlots slots: same-address atomics serialise
     if (tid == 0 && pbox[0] != INT_MAX) {
-        atomicMin(&ws->bb[0], pbox[0]); atomicMin(&ws->bb[1], pbox[1]);
-        atomicMin(&ws->bb[2], pbox[2]); atomicMin(&ws->bb[3], pbox[3]);
+        int *b = ws->bbs[blockIdx.x % kBBSlots];
+        atomicMin(&b[0], pbox[0]); atomicMin(&b[1], pbox[1]);
+        atomicMin(&b[2], pbox[2]); atomicMin(&b[3], pbox[3]);
     }
     if (wbox[0] == INT_MAX) return;
     const int ww = wbox[2] - wbox[0] + 1, wh = wbox[3] - wbox[1] + 1;
@@ -913,8 +920,20 @@ __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const W
 // window move decision: keep the bunch's leading edge `lead` cells from the
 // window front and its centre near the window centre in y (at most one cell
 // per axis per step, as moveMesh, main.cpp:562-652).
+// one wavefront: reduce and clear the kBBSlots particle-box slots, then
+// lane 0 decides the move
 __global__ void k_move(WinState *ws, int W, int H, int Wp, int Hp, int lead)
 {
+    const int t = threadIdx.x;
+    int b[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        b[c] = t < kBBSlots ? ws->bbs[t][c] : INT_MAX;
+        if (t < kBBSlots) ws->bbs[t][c] = INT_MAX;
+        b[c] = hu::wave_min(b[c]);
+    }
+    if (t != 0) return;
+    for (int c = 0; c < 4; ++c) ws->bb[c] = b[c];
     const int bmaxx = -ws->bb[2] - ws->wx0;
     const int bminy = ws->bb[1] - ws->wy0, bmaxy = -ws->bb[3] - ws->wy0;
     int mi = bmaxx > W - 1 - lead ? 1 : 0;
@@ -1217,6 +1236,8 @@ __global__ void k_state_init(WinState *ws, int wx0, int wy0)
 {
     ws->wx0 = wx0; ws->wy0 = wy0; ws->rx = 0; ws->ry = 0; ws->mi = 0; ws->mj = 0;
     for (int k = 0; k < 4; ++k) ws->bb[k] = INT_MAX;
+    for (int q = 0; q < kBBSlots; ++q)
+        for (int k = 0; k < 4; ++k) ws->bbs[q][k] = INT_MAX;
     for (int c = 0; c < 2; ++c)
         for (int k = 0; k < 4; ++k) ws->jbox[c][k] = INT_MAX;
     ws->err = 0;
@@ -1242,8 +1263,9 @@ __global__ void k_bbox_init(Parts<float> pf, Parts<double> pd, bool isf, WinStat
 #pragma unroll
     for (int c = 0; c < 4; ++c) b[c] = hu::wave_min(b[c]);
     if ((threadIdx.x & 63) == 0 && b[0] != INT_MAX) {
+        int *s = ws->bbs[(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kBBSlots];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) atomicMin(&ws->bb[c], b[c]);
+        for (int c = 0; c < 4; ++c) atomicMin(&s[c], b[c]);
     }
 }
 
@@ -1445,7 +1467,7 @@ void one_step(csrst_mw *m, bool timed)
     if (m->n > 0 && m->p.statModulus > 0 && m->steps % m->p.statModulus == m->p.statModulus - 1) wake<T>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[4], m->s));
     if (m->n > 0) {
-        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, m->lead);
+        mw::k_move<<<1, 64, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, m->lead);
         const int nf = 2 * (m->H + 2 * mw::G) + m->W + 2 * mw::G;
         hipLaunchKernelGGL((mw::k_fresh<T>), dim3((nf + mw::kB - 1) / mw::kB), dim3(mw::kB), 0, m->s, f2, m->geo, m->ws.p);
         HU_LAUNCH_CHECK();
@@ -1567,6 +1589,8 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         m->Wp = p->W + 2 * mw::G; m->Hp = p->H + 2 * mw::G;
         m->plane = (size_t)m->Wp * m->Hp;
         const size_t rs = m->f32 ? 4 : 8;
+        // the push gathers address a field plane with 32-bit byte offsets
+        if (m->plane * rs > 0xffffffffull) throw Error(CSRST_EINVAL, "window plane exceeds 4 GiB");
         m->fieldsBuf.alloc(m->plane * 12 * rs);   // ex ey hz jx jy ex' hz' + 5 cut planes
         m->fieldsBuf.zero(m->s);
         if (p->pml < 0 || 2 * p->pml > p->W - 8) throw Error(CSRST_EINVAL, "pml must be in [0, (W - 8) / 2]");
@@ -1679,7 +1703,7 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
         if (f32) pf = m->parts<float>(m->partBuf); else pd = m->parts<double>(m->partBuf);
         mw::k_bbox_init<<<(n + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(pf, pd, f32, m->ws.p);
         // initial footprint = particle box +- 4 (k_move convention)
-        mw::k_move<<<1, 1, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, -1);
+        mw::k_move<<<1, 64, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, -1);
         HU_LAUNCH_CHECK();
         // k_move with a negative lead makes no move; reset the counters
         mw::WinState h;

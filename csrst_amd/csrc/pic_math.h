@@ -157,7 +157,7 @@ PM_HD bool crossings(Crossings<T> &c, vec2<T> o, vec2<T> e, vec2<T> sp, bool dua
 // in units of the spacing, relative to the patch origin.
 template <class T>
 struct Stencil {
-    long long I0, J0;
+    int I0, J0;          // first node; cell indices fit 32 bits
     int n;
     T wx[4], wy[4];
 };
@@ -169,7 +169,7 @@ PM_HD Stencil<T> edge_stencil(int order, int lc, vec2<T> ppos)
     Stencil<T> s;
     if (order == 0) {  // ScatterGatherImpl.hh:86-102
         T ax = lc == 0 ? T(0.5) : T(0), ay = lc == 1 ? T(0.5) : T(0);
-        long long lx = (long long)floor_(ppos.x + ax), ly = (long long)floor_(ppos.y + ay);
+        int lx = (int)floor_(ppos.x + ax), ly = (int)floor_(ppos.y + ay);
         if (lc == 0) lx -= 1; else ly -= 1;
         T gx = (T)lx, gy = (T)ly;
         if (lc == 0) gx += T(0.5); else gy += T(0.5);
@@ -179,7 +179,7 @@ PM_HD Stencil<T> edge_stencil(int order, int lc, vec2<T> ppos)
         s.wy[0] = 1 - dy; s.wy[1] = dy;
     } else if (order == 1) {  // ScatterGatherImpl.hh:218-238
         T ax = lc == 0 ? T(0) : T(0.5), ay = lc == 1 ? T(0) : T(0.5);
-        long long lx = (long long)floor_(ppos.x + ax), ly = (long long)floor_(ppos.y + ay);
+        int lx = (int)floor_(ppos.x + ax), ly = (int)floor_(ppos.y + ay);
         T gx = (T)lx, gy = (T)ly;
         if (lc == 0) gx += T(0.5); else gy += T(0.5);
         const T dx = ppos.x - gx, dy = ppos.y - gy;
@@ -192,7 +192,7 @@ PM_HD Stencil<T> edge_stencil(int order, int lc, vec2<T> ppos)
         s.wy[0] = T(1.0) - (s.wy[1] + s.wy[2]);
     } else {  // ScatterGatherImpl.hh:424-447
         T ax = lc == 0 ? T(0.5) : T(0), ay = lc == 1 ? T(0.5) : T(0);
-        long long lx = (long long)floor_(ppos.x + ax), ly = (long long)floor_(ppos.y + ay);
+        int lx = (int)floor_(ppos.x + ax), ly = (int)floor_(ppos.y + ay);
         if (lc == 0) lx -= 1; else ly -= 1;
         T gx = (T)lx, gy = (T)ly;
         if (lc == 0) gx += T(0.5); else gy += T(0.5);
@@ -218,13 +218,13 @@ PM_HD Stencil<T> cell_stencil(int order, vec2<T> ppos)
 {
     Stencil<T> s;
     if (order == 0) {  // ScatterGatherImpl.hh:118-132
-        long long lx = (long long)floor_(ppos.x - T(0.5)), ly = (long long)floor_(ppos.y - T(0.5));
+        int lx = (int)floor_(ppos.x - T(0.5)), ly = (int)floor_(ppos.y - T(0.5));
         const T dx = ppos.x - ((T)lx + T(0.5)), dy = ppos.y - ((T)ly + T(0.5));
         s.I0 = lx; s.J0 = ly; s.n = 2;
         s.wx[0] = 1 - dx; s.wx[1] = dx;
         s.wy[0] = 1 - dy; s.wy[1] = dy;
     } else if (order == 1) {  // ScatterGatherImpl.hh:269-286
-        long long lx = (long long)floor_(ppos.x), ly = (long long)floor_(ppos.y);
+        int lx = (int)floor_(ppos.x), ly = (int)floor_(ppos.y);
         const T dx = ppos.x - ((T)lx + T(0.5)), dy = ppos.y - ((T)ly + T(0.5));
         s.I0 = lx - 1; s.J0 = ly - 1; s.n = 3;
         s.wx[1] = T(0.75) - dx * dx;
@@ -234,7 +234,7 @@ PM_HD Stencil<T> cell_stencil(int order, vec2<T> ppos)
         s.wy[2] = T(0.5) * (T(0.5) + dy) * (T(0.5) + dy);
         s.wy[0] = T(1.0) - (s.wy[1] + s.wy[2]);
     } else {  // ScatterGatherImpl.hh:486-505
-        long long lx = (long long)floor_(ppos.x - T(0.5)), ly = (long long)floor_(ppos.y - T(0.5));
+        int lx = (int)floor_(ppos.x - T(0.5)), ly = (int)floor_(ppos.y - T(0.5));
         const T dx = ppos.x - ((T)lx + T(0.5)), dy = ppos.y - ((T)ly + T(0.5));
         const T dx2 = dx * dx, dy2 = dy * dy;
         const T tt = T(2.0 / 3.0), os = T(1.0 / 6.0);
