@@ -1090,7 +1090,11 @@ __global__ void k_sort_keys(Parts<T> p, const WinState *ws, int Wk, unsigned *ke
     if (k >= p.n) return;
     const int i = p.gx[k] - ws->wx0 + 8, j = p.gy[k] - ws->wy0 + 8;
     const unsigned tile = (unsigned)(j >> 2) * (unsigned)(Wk >> 4) + (unsigned)(i >> 4);
-    keys[k] = (tile << 6) | ((unsigned)(j & 3) << 4) | (unsigned)(i & 15);
+    // lowest bits: the half of the cell the particle is in.  The second push
+    // of a step starts half a step on, where a cell's particles have split by
+    // these halves; runs of consecutive particles then still share a cell.
+    const unsigned hx = p.fx[k] >= T(0.5) ? 1u : 0u, hy = p.fy[k] >= T(0.5) ? 1u : 0u;
+    keys[k] = (tile << 8) | ((unsigned)(j & 3) << 6) | ((unsigned)(i & 15) << 2) | (hy << 1) | hx;
     idx[k] = k;
 }
 
@@ -1594,6 +1598,9 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         m->fieldsBuf.alloc(m->plane * 12 * rs);   // ex ey hz jx jy ex' hz' + 5 cut planes
         m->fieldsBuf.zero(m->s);
         if (p->pml < 0 || 2 * p->pml > p->W - 8) throw Error(CSRST_EINVAL, "pml must be in [0, (W - 8) / 2]");
+        // sort keys: 24 bits of 16x4-cell tile index above 8 bits of cell and half-cell
+        if ((long long)((p->H + 16 + 3) / 4) * (long long)((p->W + 16 + 15) / 16) >= (1ll << 24))
+            throw Error(CSRST_EINVAL, "window too large for the 32-bit particle sort key");
         m->npml = p->pml;
         if (m->npml) {
             // Convolutional PML (kappa 1, alpha 0): psi <- b psi + (b - 1) D once
