@@ -152,11 +152,36 @@ struct PushConst {
     T q;
 };
 
-// wave-wide OR / MAX of an int (64 lanes)
+// Wave-wide (64 lanes) integer reductions with DPP row shifts and row
+// broadcasts (no LDS traffic): lane 63 ends with the result, which readlane
+// returns to every lane.  A lane without a DPP source reads the identity.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int identity, int v)
+{
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xf, false);
+}
+template <class Op>
+__device__ __forceinline__ int wave_reduce_i(int v, int identity, Op op)
+{
+    v = op(v, dpp_i<0x111, 0xf>(identity, v));   // row_shr:1
+    v = op(v, dpp_i<0x112, 0xf>(identity, v));   // row_shr:2
+    v = op(v, dpp_i<0x114, 0xf>(identity, v));   // row_shr:4
+    v = op(v, dpp_i<0x118, 0xf>(identity, v));   // row_shr:8
+    v = op(v, dpp_i<0x142, 0xa>(identity, v));   // row_bcast:15
+    v = op(v, dpp_i<0x143, 0xc>(identity, v));   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_i(int v)
+{
+    return wave_reduce_i(v, INT_MAX, [](int a, int b) { return a < b ? a : b; });
+}
+__device__ __forceinline__ int wave_max_i(int v)
+{
+    return wave_reduce_i(v, INT_MIN, [](int a, int b) { return a > b ? a : b; });
+}
 __device__ __forceinline__ int wave_or(int v)
 {
-    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-    return v;
+    return wave_reduce_i(v, 0, [](int a, int b) { return a | b; });
 }
 
 // Kick and rotation of PartBunch.cpp:366-383 on a momentum stored as
@@ -460,7 +485,6 @@ template <class T, int ORDER, int COMP>
 __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
     __shared__ T acc[kTH * kTW];
-    __shared__ int wbox[4];          // written-node box of the accumulation tile
     __shared__ int pbox[4];          // particle cell box of the chunk (min x, min y, -max x, -max y)
     const int tid = threadIdx.x, lane = tid & 63;
     const int base = blockIdx.x * kChunk;
@@ -475,14 +499,12 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
     // sparse chunk that fall outside go to global atomics
     const int ax = p.gx[base] - 20, ay = p.gy[base] - 6;   // LDS node (0,0) = global node (ax, ay)
     if (tid == 0) {
-        wbox[0] = INT_MAX; wbox[1] = INT_MAX; wbox[2] = INT_MIN; wbox[3] = INT_MIN;
         pbox[0] = INT_MAX; pbox[1] = INT_MAX; pbox[2] = INT_MAX; pbox[3] = INT_MAX;
     }
     for (int q = tid; q < kTH * kTW; q += kB) acc[q] = T(0);
     __syncthreads();
     int err = 0;
     int bminx = INT_MAX, bminy = INT_MAX, bmaxx = INT_MIN, bmaxy = INT_MIN;
-    int tminx = INT_MAX, tminy = INT_MAX, tmaxx = INT_MIN, tmaxy = INT_MIN;
     constexpr int NX = ORDER == 0 ? (COMP == 0 ? 1 : 2) : (ORDER == 1 ? (COMP == 0 ? 2 : 3) : (COMP == 0 ? 3 : 4));
     constexpr int NY = ORDER == 0 ? (COMP == 0 ? 2 : 1) : (ORDER == 1 ? (COMP == 0 ? 3 : 2) : (COMP == 0 ? 4 : 3));
     const vec2<T> odt0 = pm::mk(c.odt_y, c.odt_x);   // S0: (1/(hy dt), 1/(hx dt)), ZerothOrderShapeFunction.cpp:224
@@ -493,7 +515,6 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         const int li = gxn - ax, lj = gyn - ay;
         if (li >= 0 && li < kTW && lj >= 0 && lj < kTH) {
             atomicAdd(&acc[lj * kTW + li], v);
-            tminx = min(tminx, li); tmaxx = max(tmaxx, li); tminy = min(tminy, lj); tmaxy = max(tmaxy, lj);
         } else {
             const int i = gxn - wx0, j = gyn - wy0;
             if (i < 0 || i > W - 1 || j < 0 || j > H - 1) err |= 1;
@@ -683,18 +704,12 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         for (int t = 0; t < NN; ++t) w[t] = slot == 0 ? accA[t] : accB[t];
         coop_add(slot == 0 ? hasA : hasB, slot == 0 ? aI : bI, slot == 0 ? aJ : bJ, w);
     }
-    // ---- written-node box of the tile, then flush the tile with one global
-    // atomic per touched node
-    tminx = hu::wave_min(tminx); tminy = hu::wave_min(tminy);
-    tmaxx = hu::wave_max(tmaxx); tmaxy = hu::wave_max(tmaxy);
-    bminx = hu::wave_min(bminx); bminy = hu::wave_min(bminy);
-    bmaxx = hu::wave_max(bmaxx); bmaxy = hu::wave_max(bmaxy);
+    // ---- chunk's particle box and error bits, then flush the tile with one
+    // global atomic per nonzero node
+    bminx = wave_min_i(bminx); bminy = wave_min_i(bminy);
+    bmaxx = wave_max_i(bmaxx); bmaxy = wave_max_i(bmaxy);
     err = wave_or(err);
     if (lane == 0) {
-        if (tminx != INT_MAX) {
-            atomicMin(&wbox[0], tminx); atomicMin(&wbox[1], tminy);
-            atomicMax(&wbox[2], tmaxx); atomicMax(&wbox[3], tmaxy);
-        }
         if (bminx != INT_MAX) {
             atomicMin(&pbox[0], bminx); atomicMin(&pbox[1], bminy);
             atomicMin(&pbox[2], -bmaxx); atomicMin(&pbox[3], -bmaxy);
@@ -709,13 +724,10 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         atomicMin(&b[0], pbox[0]); atomicMin(&b[1], pbox[1]);
         atomicMin(&b[2], pbox[2]); atomicMin(&b[3], pbox[3]);
     }
-    if (wbox[0] == INT_MAX) return;
-    const int ww = wbox[2] - wbox[0] + 1, wh = wbox[3] - wbox[1] + 1;
-    for (int q = tid; q < ww * wh; q += kB) {
-        const int li = wbox[0] + q % ww, lj = wbox[1] + q / ww;
-        const T v = acc[lj * kTW + li];
+    for (int q = tid; q < kTH * kTW; q += kB) {
+        const T v = acc[q];
         if (v == T(0)) continue;
-        const int i = ax + li - wx0, j = ay + lj - wy0;
+        const int i = ax + (q % kTW) - wx0, j = ay + (q / kTW) - wy0;
         if (i < 0 || i > W - 1 || j < 0 || j > H - 1) { atomicOr(&ws->err, 1); continue; }
         atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
     }
