@@ -785,7 +785,7 @@ __device__ __forceinline__ T ex_update(const Fields<T> &f, size_t q, uint8_t cls
 // rows (-1, H) are never updated.
 // Columns: PML = false sweeps the ncol columns from cbase; PML = true the two
 // end layers (t < npml: column t, else column W - 2 npml + t).
-template <class T, bool PML>
+template <class T, bool PML, int STRIP>
 __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const WinState *ws, int jbase, int cbase,
                                             int ncol)
 {
@@ -793,9 +793,9 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
     if (t >= ncol) return;
     const int i = PML ? (t < f.npml ? t : f.W - 2 * f.npml + t) : cbase + t;   // logical column
     const int rx = ws->rx, ry = ws->ry;
-    const int j0 = jbase + blockIdx.y * kStrip;
+    const int j0 = jbase + blockIdx.y * STRIP;
     if (j0 > f.H - 1) return;
-    const int j1 = min(j0 + kStrip - 1, f.H - 1);
+    const int j1 = min(j0 + STRIP - 1, f.H - 1);
     const int px = ringx(i, rx, f.Wp), pxp = ringx(i + 1, rx, f.Wp);
     const int jlo0 = ws->jbox[0][0] - ws->wx0, jlo1 = ws->jbox[0][1] - ws->wy0;
     const int jhi0 = -ws->jbox[0][2] - ws->wx0, jhi1 = -ws->jbox[0][3] - ws->wy0;
@@ -883,7 +883,7 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
 // Ey (TETMSolver.hh:241-249, conformal TETMSolver.cpp:117-127); clears Jy in
 // its footprint after use and Jx in the Jx footprint, leaving both current
 // planes zero for the next deposits.
-template <class T, bool PML>
+template <class T, bool PML, int STRIP>
 __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const WinState *ws, int cbase, int ncol)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const W
     const bool colInJx = i >= xlo0 && i <= xhi0;
     const bool inPml = PML && (i < f.npml || i >= f.W - f.npml);
     const T be = inPml ? f.pml[2 * f.W + i] : T(0), ae = inPml ? f.pml[3 * f.W + i] : T(0);
-    const int j0 = blockIdx.y * kStrip, j1 = min(j0 + kStrip - 1, f.H - 1);
+    const int j0 = blockIdx.y * STRIP, j1 = min(j0 + STRIP - 1, f.H - 1);
     for (int j = j0; j <= j1; ++j) {
         const size_t rb = (size_t)ringy(j, ry, f.Hp) * f.Wp;
         const size_t q = rb + px;
@@ -1460,7 +1460,10 @@ void one_step(csrst_mw *m, bool timed)
     // interior columns without PML terms; the two PML layers in a launch of their own
     const int np = m->npml, nin = m->W - 2 * np;
     const dim3 gs((nin + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
-    const dim3 gp((2 * np + mw::kB - 1) / mw::kB, (m->H + mw::kStrip - 1) / mw::kStrip);
+    // the PML columns are few: one wavefront of columns per workgroup and
+    // short strips, so that the launch still spreads over the chip
+    constexpr int kPmlStrip = 8;
+    const dim3 gp((2 * np + 63) / 64, (m->H + kPmlStrip - 1) / kPmlStrip);
     if (m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0) {
         if (timed) HU_CHECK(hipEventRecord(m->ev[6], m->s));
         sort_particles<T>(m);
@@ -1469,16 +1472,16 @@ void one_step(csrst_mw *m, bool timed)
     if (timed) HU_CHECK(hipEventRecord(m->ev[0], m->s));
     if (m->n > 0) push_dispatch<T, 0>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[1], m->s));
-    hipLaunchKernelGGL((mw::k_heh<T, false>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, np, nin);
-    if (np) hipLaunchKernelGGL((mw::k_heh<T, true>), gp, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, 0, 2 * np);
+    hipLaunchKernelGGL((mw::k_heh<T, false, mw::kStrip>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, np, nin);
+    if (np) hipLaunchKernelGGL((mw::k_heh<T, true, kPmlStrip>), gp, dim3(64), 0, m->s, f, sc, m->ws.p, 0, 0, 2 * np);
     HU_LAUNCH_CHECK();
     m->cur ^= 1;
     const mw::Fields<T> f2 = m->fields<T>();
     if (timed) HU_CHECK(hipEventRecord(m->ev[2], m->s));
     if (m->n > 0) push_dispatch<T, 1>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[3], m->s));
-    hipLaunchKernelGGL((mw::k_ey<T, false>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, np, nin);
-    if (np) hipLaunchKernelGGL((mw::k_ey<T, true>), gp, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, 0, 2 * np);
+    hipLaunchKernelGGL((mw::k_ey<T, false, mw::kStrip>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, np, nin);
+    if (np) hipLaunchKernelGGL((mw::k_ey<T, true, kPmlStrip>), gp, dim3(64), 0, m->s, f2, sc, m->ws.p, 0, 2 * np);
     HU_LAUNCH_CHECK();
     if (m->n > 0 && m->p.statModulus > 0 && m->steps % m->p.statModulus == m->p.statModulus - 1) wake<T>(m);
     if (timed) HU_CHECK(hipEventRecord(m->ev[4], m->s));
