@@ -264,6 +264,20 @@ __device__ __forceinline__ double wave_sum(double v)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// sum over the 4 lanes of each quad (DPP quad permutations [1,0,3,2], [2,3,0,1])
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += dpp_f<0xb1, 0xf>(v);
+    v += dpp_f<0x4e, 0xf>(v);
+    return v;
+}
+__device__ __forceinline__ double quad_sum(double v)
+{
+    v += dpp_d<0xb1, 0xf>(v);
+    v += dpp_d<0x4e, 0xf>(v);
+    return v;
+}
+
 template <class T>
 __device__ __forceinline__ void atomic_add_real(T *p, T v);
 template <>
@@ -390,7 +404,7 @@ __device__ __forceinline__ void seg_s2_sep(pm::SegCurrent<T> &o, int comp, T tau
 {
     const T h = T(0.5), o3 = T(1.0 / 3.0), o4 = T(0.25), o5 = T(0.2);
     const T tp2 = tau * tau / 2;
-    const T tp3 = tp2 * tau * 2 / 3;
+    const T tp3 = tp2 * tau * T(2.0 / 3.0);   // a multiply: this form need not divide as the reference does
     const T tp4 = tp2 * tp2;
     // along the current: quadratic A2 + B2 + C2 in a; across: cubic in b
     const T al = comp == 0 ? a.x : a.y, dl = comp == 0 ? d.x : d.y;
@@ -528,9 +542,13 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
         for (int t = 0; t < NN; ++t) node_add(I0 + (t % NX), J0 + (t / NX), w[t]);
     };
     // Wavefront-cooperative add of the lanes' stencils (lanes with `mine`
-    // set), one pass per distinct anchor: the NN node sums are formed with DPP
-    // and handed to lanes 0 .. NN-1, which add them in one step.  Called in
-    // wave-uniform control flow (every lane active).
+    // set), one pass per distinct anchor.  The NN x 64 values go through the
+    // wave's LDS scratch (node-major, conflict-free writes); lane l < 4 NN
+    // sums 16 of node l / 4's values and a quad sum completes it, then lanes
+    // 4t add node t into the tile in one step.  Called in wave-uniform control
+    // flow (every lane active).
+    __shared__ T red[(kB / 64) * NN * 64];
+    T *rw = red + (tid >> 6) * (NN * 64);
     auto coop_add = [&](bool mine0, int sI, int sJ, const T *w) {
         unsigned long long rem = __ballot(mine0);
         while (rem) {
@@ -538,13 +556,21 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
             const int lI = __builtin_amdgcn_readlane(sI, lead), lJ = __builtin_amdgcn_readlane(sJ, lead);
             const bool mine = mine0 && sI == lI && sJ == lJ;
             rem &= ~__ballot(mine);
-            T v = T(0);
 #pragma unroll
-            for (int t = 0; t < NN; ++t) {
-                const T st = wave_sum(mine ? w[t] : T(0));
-                v = lane == t ? st : v;
+            for (int t = 0; t < NN; ++t) rw[t * 64 + lane] = mine ? w[t] : T(0);
+            __builtin_amdgcn_wave_barrier();
+            T v = T(0);
+            if (lane < 4 * NN) {
+                const T *src = rw + (lane >> 2) * 64 + (lane & 3) * 16;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v += src[k];
             }
-            if (lane < NN) node_add(lI + (lane % NX), lJ + (lane / NX), v);
+            v = quad_sum(v);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 4 * NN && (lane & 3) == 0) {
+                const int t = lane >> 2;
+                node_add(lI + (t % NX), lJ + (t / NX), v);
+            }
         }
     };
     // two register accumulation slots per lane
