@@ -188,6 +188,18 @@ __device__ __forceinline__ int wave_or(int v)
 // P = p0 + dP: the increments are formed directly so that FP32 keeps the
 // resolution of dP (|dP| << |P|).  Algebraically identical to
 // P' = ((1+a^2) s + a s_perp)/(1+a^2) + kick, s = um + a (um_y, -um_x).
+// S2 (cubic B-spline) weights of the four nodes for offset d in [0, 1) from
+// the second node, in the arithmetic of ScatterGatherImpl.hh:436-446.
+template <class T>
+__device__ __forceinline__ void s2_weights(T d, T (&w)[4])
+{
+    const T d2 = d * d, tt = T(2.0 / 3.0), os = T(1.0 / 6.0);
+    w[1] = tt - d2 * (1 - T(0.5) * d);
+    w[2] = tt - T(0.5) * (1 - d) * (1 - d2);
+    w[3] = os * d2 * d;
+    w[0] = T(1.0) - (w[1] + w[2] + w[3]);
+}
+
 // Quotient of the split-precision form: FP32 uses the hardware reciprocal
 // (1 ulp) instead of the IEEE division sequence; FP64 divides exactly.
 __device__ __forceinline__ float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -637,9 +649,38 @@ __global__ void __launch_bounds__(kB) k_push(Parts<T> p, Fields<T> f, Geo g, Pus
                         return *reinterpret_cast<const T *>(pb + (rr[b] + cc[a]));
                     });
                 };
-                const T ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
-                const T ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
-                const T hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
+                T ex, ey, hf;
+                if constexpr (ORDER == 2) {
+                    // The three S2 stencils use two weight sets per axis: at the
+                    // offset from the cell corner (Ex's y, Ey's x) and from the
+                    // nearest cell centre (Ex's x, Ey's y, both of Hz's)
+                    // (ScatterGatherImpl.hh:424-447, 486-505 with fx, fy in [0, 1)).
+                    pm::Stencil<T> sx, sy;
+                    const int lx = (int)pm::floor_(pp.x - T(0.5)), ly = (int)pm::floor_(pp.y - T(0.5));
+                    T wxu[4], wyu[4];
+                    s2_weights(pp.x - ((T)lx + T(0.5)), sx.wx);
+                    s2_weights(pp.y - ((T)ly + T(0.5)), sy.wy);
+                    s2_weights(pp.x, wxu);
+                    s2_weights(pp.y, wyu);
+                    pm::Stencil<T> st;
+                    st.n = 4;
+                    st.I0 = lx - 1; st.J0 = -1;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) { st.wx[a] = sx.wx[a]; st.wy[a] = wyu[a]; }
+                    ex = gat(f.ex, st);
+                    st.I0 = -1; st.J0 = ly - 1;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) { st.wx[a] = wxu[a]; st.wy[a] = sy.wy[a]; }
+                    ey = gat(f.ey, st);
+                    st.I0 = lx - 1;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) st.wx[a] = sx.wx[a];
+                    hf = gat(f.hz, st);
+                } else {
+                    ex = gat(f.ex, pm::edge_stencil(ORDER, 0, pp));
+                    ey = gat(f.ey, pm::edge_stencil(ORDER, 1, pp));
+                    hf = gat(f.hz, pm::cell_stencil(ORDER, pp));
+                }
                 const double X = g.org[0] + ((double)gx + (double)fx) * g.h[0];
                 const double Y = g.org[1] + ((double)gy + (double)fy) * g.h[1];
                 const T bext = (T)geom::external_bz(g.bend, X, Y);
