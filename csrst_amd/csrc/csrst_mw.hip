@@ -916,19 +916,35 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
     T exnp = 0, eyp_prev = 0, ey_prev = 0, h1pp = 0;
     uint8_t clp = 0;
     size_t qprev = 0;
+    // the march's loads run one row ahead of its arithmetic
+    struct RowIn {
+        T ey, eyp, hz, jv, psi, exnext;
+        uint8_t cl;
+    };
+    auto fetch = [&](int j, RowIn &r) {
+        const size_t rb = row(j), q = rb + px;
+        r.ey = f.ey[q];
+        r.eyp = f.ey[rb + pxp];
+        r.hz = f.hz[q];
+        r.cl = f.cls[q];
+        r.exnext = f.ex[row(j + 1) + px];
+        r.jv = (colInJ && j >= jlo1 && j <= jhi1 && j <= f.H - 1) ? f.jx[q] : T(0);
+        r.psi = inPml ? f.psih[q] : T(0);
+    };
+    RowIn cur, nxt;
+    fetch(j0, cur);
     for (int j = j0; j <= j1 + 1; ++j) {
-        const size_t rb = row(j);
-        const size_t q = rb + px;
-        const T exo_next = f.ex[row(j + 1) + px];
-        const T ey = f.ey[q], eyp = f.ey[rb + pxp];
-        const uint8_t cl = f.cls[q];
+        if (j + 1 <= j1 + 1) fetch(j + 1, nxt);
+        const size_t q = row(j) + px;
+        const T exo_next = cur.exnext;
+        const T ey = cur.ey, eyp = cur.eyp;
+        const uint8_t cl = cur.cl;
         T h1, exn, psi1 = 0;
         if (j <= f.H - 1) {
-            h1 = hup(q, cl, f.hz[q], ey, eyp, exo, exo_next, inPml ? f.psih[q] : T(0), psi1, true);
-            const T jv = (colInJ && j >= jlo1 && j <= jhi1) ? f.jx[q] : T(0);
-            exn = ex_update(f, q, cl, exo, h1, h1p, jv, c);
+            h1 = hup(q, cl, cur.hz, ey, eyp, exo, exo_next, cur.psi, psi1, true);
+            exn = ex_update(f, q, cl, exo, h1, h1p, cur.jv, c);
         } else {
-            h1 = f.hz[q];
+            h1 = cur.hz;
             exn = exo;
         }
         if (j > j0) {
@@ -944,6 +960,7 @@ __global__ void __launch_bounds__(kB) k_heh(Fields<T> f, SweepConst<T> c, const 
         exnp = exn;
         exo = exo_next;
         ey_prev = ey; eyp_prev = eyp; clp = cl; qprev = q;
+        cur = nxt;
     }
 }
 
@@ -967,30 +984,51 @@ __global__ void __launch_bounds__(kB) k_ey(Fields<T> f, SweepConst<T> c, const W
     const bool inPml = PML && (i < f.npml || i >= f.W - f.npml);
     const T be = inPml ? f.pml[2 * f.W + i] : T(0), ae = inPml ? f.pml[3 * f.W + i] : T(0);
     const int j0 = blockIdx.y * STRIP, j1 = min(j0 + STRIP - 1, f.H - 1);
-    for (int j = j0; j <= j1; ++j) {
-        const size_t rb = (size_t)ringy(j, ry, f.Hp) * f.Wp;
-        const size_t q = rb + px;
-        const uint8_t cl = f.cls[q];
-        const T d = f.hz[q] - f.hz[rb + pxm];
-        T jv = 0;
-        if (colInJ && j >= jlo1 && j <= jhi1) { jv = f.jy[q]; f.jy[q] = 0; }
-        if (colInJx && j >= xlo1 && j <= xhi1) f.jx[q] = 0;
-        if (cl == 1) {
-            T dd = d;
-            if (PML && inPml) {
-                const T psi = be * f.psie[q] + ae * d;
-                f.psie[q] = psi;
-                dd = d + psi;
+    // rows are independent: each group of four issues all its loads before
+    // any arithmetic or store
+    constexpr int G4 = 4;
+    for (int jb = j0; jb <= j1; jb += G4) {
+        size_t qv[G4];
+        uint8_t clv[G4];
+        T dv[G4], eyv[G4], jvv[G4], psv[G4];
+#pragma unroll
+        for (int u = 0; u < G4; ++u) {
+            const int j = min(jb + u, j1);
+            const size_t rb = (size_t)ringy(j, ry, f.Hp) * f.Wp;
+            const size_t q = rb + px;
+            qv[u] = q;
+            clv[u] = f.cls[q];
+            dv[u] = f.hz[q] - f.hz[rb + pxm];
+            eyv[u] = f.ey[q];
+            jvv[u] = (colInJ && j >= jlo1 && j <= jhi1) ? f.jy[q] : T(0);
+            psv[u] = (PML && inPml) ? f.psie[q] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < G4; ++u) {
+            const int j = jb + u;
+            if (j > j1) break;
+            const size_t q = qv[u];
+            const uint8_t cl = clv[u];
+            const T d = dv[u], jv = jvv[u];
+            if (colInJ && j >= jlo1 && j <= jhi1) f.jy[q] = 0;
+            if (colInJx && j >= xlo1 && j <= xhi1) f.jx[q] = 0;
+            if (cl == 1) {
+                T dd = d;
+                if (PML && inPml) {
+                    const T psi = be * psv[u] + ae * d;
+                    f.psie[q] = psi;
+                    dd = d + psi;
+                }
+                f.ey[q] = eyv[u] - (c.sey * dd + c.dteps * jv);
+            } else if (cl == 2 && f.cut[f.plane + q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea)) {
+                T dd = d;
+                if (PML && inPml) {
+                    const T psi = be * psv[u] + ae * d;
+                    f.psie[q] = psi;
+                    dd = d + psi;
+                }
+                f.ey[q] = eyv[u] + c.ce * (-dd * c.ihx - jv);
             }
-            f.ey[q] = f.ey[q] - (c.sey * dd + c.dteps * jv);
-        } else if (cl == 2 && f.cut[f.plane + q] > T(pm::kSpaceEps) && f.cut[2 * f.plane + q] >= T(pm::kMinArea)) {
-            T dd = d;
-            if (PML && inPml) {
-                const T psi = be * f.psie[q] + ae * d;
-                f.psie[q] = psi;
-                dd = d + psi;
-            }
-            f.ey[q] = f.ey[q] + c.ce * (-dd * c.ihx - jv);
         }
     }
 }
