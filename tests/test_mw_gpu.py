@@ -112,8 +112,9 @@ def test_bunch_advances_at_c(gpu):
     assert abs(adv / (30 * 299792458.0 * dt) - 1.0) < 1e-3
 
 
-@pytest.mark.parametrize("order", [0, 1, 2])
-def test_first_step_ex_is_deposit(gpu, order):
+@pytest.mark.parametrize("order,fp32,tol", [(0, False, 1e-12), (1, False, 1e-12), (2, False, 1e-12),
+                                            (1, True, 2e-6), (2, True, 2e-6)])
+def test_first_step_ex_is_deposit(gpu, order, fp32, tol):
     """From zero fields, after one step Ex = -c dt Z0 Jx exactly (the first
     half step of H is zero, the second does not feed back into Ex), with Jx
     the oracle deposit of the first push R0 -> R0 + 2 (0.5 c dt) P/gamma."""
@@ -124,7 +125,7 @@ def test_first_step_ex_is_deposit(gpu, order):
     R = np.stack([0.02 + rng.normal(0, 0.3e-3, n), 0.0064 + rng.normal(0, 0.4e-3, n)], 1)
     P = np.stack([np.full(n, 99.995), np.zeros(n)], 1)
     Q = np.full(n, -1e-22)
-    m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=False, sortEvery=0, statModulus=0, **GEOM)
+    m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
     m.init_bunch(R, P, Q)
     inf = m.info()
     wx0, wy0 = inf["pos"]
@@ -147,7 +148,8 @@ def test_first_step_ex_is_deposit(gpu, order):
     scale = np.abs(expect).max()
     d = np.abs(Ex - expect)
     j, i = np.unravel_index(np.argmax(d), d.shape)
-    assert d.max() < 1e-12 * scale, "max |Ex - expect| = %.3e of %.3e at (%d, %d): %r vs %r" % (
+    print("order %d fp32 %s: max |Ex - expect| / max |expect| = %.2e" % (order, fp32, d.max() / scale))
+    assert d.max() < tol * scale, "max |Ex - expect| = %.3e of %.3e at (%d, %d): %r vs %r" % (
         d.max(), scale, i, j, Ex[j, i], expect[j, i])
 
 
