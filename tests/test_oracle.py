@@ -214,3 +214,46 @@ def test_two_grid_step_window_follows_bunch():
     p1 = s.info()["pos"]
     assert 8 <= p1[0] - p0[0] <= 10
     assert np.isfinite(s.field("mE")).all() and np.abs(s.field("mJ")).max() > 0
+
+
+def test_split_field_step_moves_div_e():
+    """The reference's split step (TETMSolver.hh:110-266) updates Ex with H
+    after the first half step and Ey with H after the second: the divergence
+    of the curl terms does not cancel, so with J = 0 div E changes by far more
+    than round-off in bulk cells.  A standard Yee step (both E components from
+    one H) would leave it unchanged; this pins the reference's scheme."""
+    from scipy.ndimage import binary_erosion
+    geo = dict(lenBefore=0.05, lenAfter=0.01, width=0.0128, phiDeg=10.0, ekinMeV=50.59, bz=0.1704)
+    nx, ny = 1100, 120
+    s = O.Sim(Nx=nx, Ny=ny, order=0, statModulus=0, **geo)
+    b = O.Bend(Nx=nx, Ny=ny, **geo)
+    hx, hy = b.hr
+    rng = np.random.default_rng(5)
+    mask = b.mask().astype(bool)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    E = np.zeros((2, ny + 2, nx + 2))
+    H = np.zeros((2, ny + 2, nx + 2))
+    for arr, k in ((E, 0), (E, 1), (H, 0)):
+        f = np.zeros((ny, nx))
+        for _ in range(4):
+            kx, ky = rng.uniform(0.02, 0.2, 2)
+            f += rng.normal() * np.sin(kx * xx + rng.uniform(0, 6)) * np.cos(ky * yy + rng.uniform(0, 6))
+        arr[k, 1:-1, 1:-1] = np.where(mask, f * 1e5, 0.0)
+    H[0, 1:-1, 1:-1] *= 1e-3
+    H[1, 1:-1, 1:-1] = H[0, 1:-1, 1:-1]
+    s.set_field("E", E)
+    s.set_field("H", H)
+
+    def div(F):
+        d = np.zeros((ny, nx))
+        d[:, 1:] += (F[0, 1:-1, 2:-1] - F[0, 1:-1, 1:-2]) / hx
+        d[1:, :] += (F[1, 2:-1, 1:-1] - F[1, 1:-2, 1:-1]) / hy
+        return d
+
+    inner = binary_erosion(mask, iterations=4)
+    d0 = div(s.field("E"))
+    s.field_steps(1)
+    E1 = s.field("E")
+    change = np.abs(div(E1) - d0)[inner].max() / (np.abs(E1[0]).max() / hx)
+    print("div E change in one field step: %.2e of max|E|/hx" % change)
+    assert change > 1e-5
