@@ -39,6 +39,7 @@
 #include "hip_util.h"
 #include "pic_math.h"
 #include "poisson_mg.h"
+#include "beam_stats.h"
 
 using hu::DBuf;
 using hu::Error;
@@ -1423,6 +1424,8 @@ struct csrst_mw {
     size_t cubBytes = 0;
     DBuf<mw::DD> partial, wacc;
     DBuf<double> mom, shift;
+    DBuf<bs::DD> bsPart;   // beam statistics partials
+    DBuf<double> bsLim;
     int n = 0;
     double q = 0;
     int nsWake = 0;
@@ -1690,6 +1693,23 @@ void mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid)
     HU_LAUNCH_CHECK();
     HU_CHECK(hipStreamSynchronize(m->s));
 }
+
+namespace {
+// particle source of the beam statistics: split positions and momentum
+// deviations, as csrst_mw_download_particles forms R and P
+template <class T>
+struct MwBeamSrc {
+    mw::Parts<T> p;
+    double o0, o1, h0, h1, p00, p01;
+    __device__ void get(int k, double &x, double &y, double &a, double &b) const
+    {
+        x = o0 + ((double)p.gx[k] + (double)p.fx[k]) * h0;
+        y = o1 + ((double)p.gy[k] + (double)p.fy[k]) * h1;
+        a = p00 + (double)p.px[k];
+        b = p01 + (double)p.py[k];
+    }
+};
+}  // namespace
 
 extern "C" {
 
@@ -2012,6 +2032,33 @@ int csrst_mw_download_fields(csrst_mw *m, double *out, int cap, int *count)
                     out[(size_t)pl * m->plane + (size_t)(j + mw::G) * m->Wp + (i + mw::G)] = v;
                 }
         }
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+
+int csrst_mw_beam_stats(csrst_mw *m, double *out)
+{
+    try {
+        if (!m || !m->ready || !out) throw Error(CSRST_ESTATE, "not initialised");
+        if (!m->bsPart.p) {
+            m->bsPart.alloc((size_t)bs::kBlocks * 11);
+            m->bsLim.alloc((size_t)bs::kBlocks * 4);
+        }
+        mw::WinState h;
+        HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
+        HU_CHECK(hipStreamSynchronize(m->s));
+        double raw[bs::kRaw];
+        const double *o = m->geo.org, *hh = m->geo.h;
+        if (m->f32)
+            bs::raw_stats(MwBeamSrc<float>{m->parts<float>(m->partBuf), o[0], o[1], hh[0], hh[1], h.p0[0], h.p0[1]},
+                          m->n, m->s, m->bsPart.p, m->bsLim.p, raw);
+        else
+            bs::raw_stats(MwBeamSrc<double>{m->parts<double>(m->partBuf), o[0], o[1], hh[0], hh[1], h.p0[0], h.p0[1]},
+                          m->n, m->s, m->bsPart.p, m->bsLim.p, raw);
+        bs::parameters(raw, out);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return mw_fail(m, e);

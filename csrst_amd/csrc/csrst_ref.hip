@@ -27,6 +27,7 @@
 #include "hip_util.h"
 #include "pic_math.h"
 #include "poisson_mg.h"
+#include "beam_stats.h"
 
 using hu::DBuf;
 using hu::Error;
@@ -713,6 +714,8 @@ struct csrst_ctx {
     // diagnostics
     DBuf<DD> partial, wacc;
     DBuf<double> mom;
+    DBuf<bs::DD> bsPart;   // beam statistics partials
+    DBuf<double> bsLim;
     // Poisson start: node classes and resetOutsideDomain codes
     DBuf<uint8_t> pcls, preset;
     int steps = 0;
@@ -1622,6 +1625,33 @@ int csrst_wake(csrst_ctx *c, double *out, int cap, int *count)
             HU_CHECK(hipMemcpy(h.data(), c->wacc.p, ns * sizeof(DD), hipMemcpyDeviceToHost));
             for (int k = 0; k < ns; ++k) out[k] = h[k].hi + h[k].lo;
         }
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// particle source of the beam statistics: the FP64 SoA bunch
+struct BeamSrc {
+    const double *rx, *ry, *px, *py;
+    __device__ void get(int k, double &x, double &y, double &a, double &b) const
+    {
+        x = rx[k]; y = ry[k]; a = px[k]; b = py[k];
+    }
+};
+
+int csrst_beam_stats(csrst_ctx *c, double *out)
+{
+    try {
+        if (!c || !c->ready || !out) throw Error(CSRST_ESTATE, "not initialised");
+        if (!c->bsPart.p) {
+            c->bsPart.alloc((size_t)bs::kBlocks * 11);
+            c->bsLim.alloc((size_t)bs::kBlocks * 4);
+        }
+        double raw[bs::kRaw];
+        bs::raw_stats(BeamSrc{c->parts.rx, c->parts.ry, c->parts.px, c->parts.py}, c->n, c->s, c->bsPart.p,
+                      c->bsLim.p, raw);
+        bs::parameters(raw, out);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);

@@ -85,6 +85,14 @@ class MovingWindowStep:
         check(_lib.lib().csrst_mw_download_particles(self.h, dptr(R), dptr(P), C.c_int(self.n)), self.h)
         return R, P
 
+    def calcBeamParameters(self):
+        """PartBunch::calcBeamParameters (PartBunch.cpp:584-657) of the bunch, as a
+        dict keyed by csrst_amd.stats.FIELDS."""
+        from .stats import N_STATS, as_dict
+        out = np.zeros(N_STATS)
+        check(_lib.lib().csrst_mw_beam_stats(self.h, dptr(out)), self.h)
+        return as_dict(out)
+
     def wake(self):
         L = _lib.lib()
         cnt = C.c_int(0)

@@ -169,3 +169,11 @@ class TwoGridStep:
         out = np.zeros(6)
         check(_lib.lib().csrst_diag(self.h, dptr(out)), self.h)
         return out
+
+    def calcBeamParameters(self):
+        """PartBunch::calcBeamParameters (PartBunch.cpp:584-657) of the current bunch,
+        as a dict keyed by csrst_amd.stats.FIELDS."""
+        from .stats import N_STATS, as_dict
+        out = np.zeros(N_STATS)
+        check(_lib.lib().csrst_beam_stats(self.h, dptr(out)), self.h)
+        return as_dict(out)

@@ -102,6 +102,14 @@ int csrst_info(csrst_ctx *c, double *info);
 int csrst_wake(csrst_ctx *c, double *out, int cap, int *count);
 /* rmean(2), pmean(2), qtotal, field energy of the global grid */
 int csrst_diag(csrst_ctx *c, double *out);
+/* Beam parameters of PartBunch::calcBeamParameters / calcMoments
+ * (PartBunch.cpp:584-756) as DataSinkCommand prints them (DataSink.cpp:91-148):
+ * out[CSRST_BEAM_STATS] = Ekin (MeV), rrms x y, prms x y, normalised emittance
+ * x y, rmean x y, pmean x y, Dy, DDy, dE (MeV), rmin x y, rmax x y, rprms x y.
+ * Moments of the positions and momenta centred on the centroid and rotated
+ * into the frame of the mean momentum; double-double sums. */
+#define CSRST_BEAM_STATS 20
+int csrst_beam_stats(csrst_ctx *c, double *out);
 
 /* ---- moving-window production step -------------------------------------- */
 
@@ -133,6 +141,8 @@ int csrst_mw_upload_fields(csrst_mw *m, const double *in, int count);
 /* planar [6][H+2][W+2] doubles: Ex, Ey, Hzx, Hzy, Jx, Jy (guards incl.) */
 int csrst_mw_download_fields(csrst_mw *m, double *out, int cap, int *count);
 int csrst_mw_download_particles(csrst_mw *m, double *R, double *P, int n);
+/* csrst_beam_stats for the moving-window bunch (PartBunch.cpp:584-756) */
+int csrst_mw_beam_stats(csrst_mw *m, double *out);
 /* info: dt, origin x, y, column offset, row offset, W, H, steps, wake samples, n */
 int csrst_mw_info(csrst_mw *m, double *info);
 int csrst_mw_wake(csrst_mw *m, double *out, int cap, int *count);

@@ -1895,6 +1895,101 @@ T *as(void *p) { return reinterpret_cast<T *>(p); }
 } // namespace
 
 /* ======================================================================= */
+// ---------------------------------------------------------------------------
+// Beam parameters: PartBunch::calcMoments (PartBunch.cpp:659-756) and
+// PartBunch::calcBeamParameters (PartBunch.cpp:584-657) on one rank, with
+// the Kahan sums in long double of utils.hh:512-521 and the reference's mix
+// of double and long double arithmetic.  out: Ekin (MeV), rrms(2), prms(2),
+// normalised emittance(2), rmean(2), pmean(2), Dy, DDy, dE, rmin(2), rmax(2),
+// rprms(2).
+namespace {
+struct KahanLD {
+    long double sum = 0, corr = 0;
+    void add(double v)
+    {
+        const long double y = v - corr;
+        const long double t = sum + y;
+        corr = (t - sum) - y;
+        sum = t;
+    }
+};
+}  // namespace
+
+void beam_parameters(const double *R, const double *P, int n, double out[20])
+{
+    for (int i = 0; i < 20; ++i) out[i] = 0;
+    if (n <= 0) return;
+    const double N = n;
+    const double m0 = 0.51099906e-03 * 1000;   // Physics.cpp:51, PartBunch.cpp:588
+    // calcMoments: centroid sums and limits
+    KahanLD sums[4];
+    double lim[4] = {R[0], R[1], R[0], R[1]};   // min x, min y, max x, max y
+    for (int k = 0; k < n; ++k) {
+        const double part[4] = {R[2 * k], P[2 * k], R[2 * k + 1], P[2 * k + 1]};
+        for (int d = 0; d < 2; ++d) {
+            lim[d] = std::min(lim[d], R[2 * k + d]);
+            lim[2 + d] = std::max(lim[2 + d], R[2 * k + d]);
+        }
+        for (int i = 0; i < 4; ++i) sums[i].add(part[i]);
+    }
+    long double centroid[4];
+    for (int i = 0; i < 4; ++i) centroid[i] = sums[i].sum;
+    const double avgR[2] = {(double)(centroid[0] / n), (double)(centroid[2] / n)};
+    const double avgP[2] = {(double)(centroid[1] / n), (double)(centroid[3] / n)};
+    const double avgmom = std::sqrt(avgP[0] * avgP[0] + avgP[1] * avgP[1]);
+    const double cosphi = avgP[0] / avgmom, sinphi = avgP[1] / avgmom;
+    KahanLD sq[16];
+    for (int k = 0; k < n; ++k) {
+        long double part[4];
+        part[1] = cosphi * P[2 * k] + sinphi * P[2 * k + 1];
+        part[3] = -sinphi * P[2 * k] + cosphi * P[2 * k + 1];
+        part[0] = R[2 * k] - avgR[0];
+        part[2] = R[2 * k + 1] - avgR[1];
+        const double tmp = (double)(cosphi * part[0] + sinphi * part[2]);
+        part[2] = -sinphi * part[0] + cosphi * part[2];
+        part[0] = tmp;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j <= i; ++j) sq[i * 4 + j].add((double)(part[i] * part[j]));
+    }
+    auto mom = [&](int i, int j) -> long double { return i >= j ? sq[i * 4 + j].sum : sq[j * 4 + i].sum; };
+    // calcBeamParameters
+    double rmean[2], pmean[2];
+    for (int i = 0; i < 2; ++i) {
+        rmean[i] = (double)(centroid[2 * i] / N);
+        pmean[i] = (double)(centroid[2 * i + 1] / N);
+    }
+    centroid[1] = pmean[0] * pmean[0] + pmean[1] * pmean[1];
+    centroid[3] = 0.0;
+    double rsq[2], psq[2], rp[2];
+    for (int i = 0; i < 2; ++i) {
+        rsq[i] = (double)mom(2 * i, 2 * i);
+        psq[i] = (double)(mom(2 * i + 1, 2 * i + 1) - N * centroid[2 * i + 1]);
+        if (psq[i] < 0) psq[i] = 0;
+        rp[i] = (double)mom(2 * i, 2 * i + 1);
+    }
+    for (int i = 0; i < 2; ++i) {
+        const double eps2 = (rsq[i] * psq[i] - rp[i] * rp[i]) / (N * N);
+        const double rpm = rp[i] / N;
+        const double rrms = std::sqrt(rsq[i] / N), prms = std::sqrt(psq[i] / N);
+        out[1 + i] = rrms;
+        out[3 + i] = prms;
+        out[5 + i] = std::sqrt(std::max(eps2, 0.0));
+        const double t = rrms * prms;
+        out[18 + i] = rpm * (t == 0 ? 0.0 : 1.0 / t);
+        out[7 + i] = rmean[i];
+        out[9 + i] = pmean[i];
+    }
+    out[11] = (double)(mom(2, 1) / N);
+    out[12] = (double)(mom(3, 1) / N);
+    double gamma = 0.0;
+    for (int k = 0; k < n; ++k) gamma += std::sqrt(1.0 + (P[2 * k] * P[2 * k] + P[2 * k + 1] * P[2 * k + 1]));
+    gamma /= N;
+    out[0] = (gamma - 1.0) * m0;
+    const double ek = out[0];
+    out[13] = out[3] * std::sqrt(ek * (ek + 2. * m0) / (1. + ek * (ek + 2. * m0) / (m0 * m0)));
+    out[14] = lim[0]; out[15] = lim[1]; out[16] = lim[2]; out[17] = lim[3];
+}
+
 extern "C" {
 
 float orc_q_rsqrt(float x) { return q_rsqrt(x); }
@@ -1970,6 +2065,13 @@ int orc_deposit(int order, int comp, double *J, int i0, int j0, int pnx, int pny
         deposit_one(order, comp, acc, v2(origin[0], origin[1]), v2(dx[0], dx[1]), dt,
                     v2(oldR[2 * k], oldR[2 * k + 1]), v2(R[2 * k], R[2 * k + 1]), Q[k]);
     return bad ? -1 : 0;
+}
+
+int orc_beam_parameters(const double *R, const double *P, int n, double *out)
+{
+    if (!R || !P || !out || n < 0) return -1;
+    beam_parameters(R, P, n, out);
+    return 0;
 }
 
 int orc_charge_density(int order, double *rho, int i0, int j0, int pnx, int pny, const double origin[2],

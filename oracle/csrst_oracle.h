@@ -49,6 +49,9 @@ int orc_deposit(int order, int comp, double *J, int i0, int j0, int pnx, int pny
                 const double origin[2], const double dx[2], double dt,
                 const double *oldR, const double *R, const double *Q, int n);
 /* Vertex charge density (divided by hx*hy); same patch convention. */
+/* PartBunch::calcMoments + calcBeamParameters (PartBunch.cpp:584-756): out[20] =
+ * Ekin, rrms(2), prms(2), emittance(2), rmean(2), pmean(2), Dy, DDy, dE, rmin(2), rmax(2), rprms(2) */
+int orc_beam_parameters(const double *R, const double *P, int n, double *out);
 int orc_charge_density(int order, double *rho, int i0, int j0, int pnx, int pny,
                        const double origin[2], const double dx[2],
                        const double *R, const double *Q, int n);

@@ -126,6 +126,22 @@ def deposit(order, comp, box, origin, dx, dt, oldR, R, Q):
     return J
 
 
+BEAM_FIELDS = ("Ekin", "rrms_x", "rrms_y", "prms_x", "prms_y", "emit_x", "emit_y", "rmean_x", "rmean_y",
+               "pmean_x", "pmean_y", "Dy", "DDy", "dE", "rmin_x", "rmin_y", "rmax_x", "rmax_y", "rprms_x",
+               "rprms_y")
+
+
+def beam_parameters(R, P):
+    """PartBunch::calcMoments + calcBeamParameters (PartBunch.cpp:584-756), as a
+    dict keyed like csrst_amd.stats.FIELDS."""
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
+    P = np.ascontiguousarray(P, np.float64).reshape(-1, 2)
+    out = np.zeros(len(BEAM_FIELDS))
+    if lib().orc_beam_parameters(_ptr(R), _ptr(P), C.c_int(len(R)), _ptr(out)) != 0:
+        raise ValueError("bad beam")
+    return {k: float(out[i]) for i, k in enumerate(BEAM_FIELDS)}
+
+
 def charge_density(order, box, origin, dx, R, Q):
     i0, j0, nx, ny = box
     rho = np.zeros((ny, nx))
