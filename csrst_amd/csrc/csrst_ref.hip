@@ -550,6 +550,20 @@ __global__ void k_energy_partial(FieldDesc E, FieldDesc H, DD *partial)
     }, n, partial);
 }
 
+// Sums of E.E and (Hzx + Hzy)^2 over the ni x nj nodes from (i0, j0) of one
+// grid: the loops of CalcEnergyCommand::execute (CalcEnergy.cpp:77-124).
+__global__ void k_energy_sums(FieldDesc E, FieldDesc H, int i0, int j0, int ni, int nj, DD *partial)
+{
+    block_dd_sum<2>([&](int k, double *v) {
+        const int i = i0 + k % ni, j = j0 + k / ni;
+        const size_t q = fidx(E, i, j);
+        const double ex = E.c0[q], ey = E.c1[q];
+        const double hz = H.c0[q] + H.c1[q];
+        v[0] = ex * ex + ey * ey;
+        v[1] = hz * hz;
+    }, ni * nj, partial);
+}
+
 // ---------------------------------------------------------------------------
 struct Rect {
     int lo[2], hi[2];
@@ -1652,6 +1666,40 @@ int csrst_beam_stats(csrst_ctx *c, double *out)
         bs::raw_stats(BeamSrc{c->parts.rx, c->parts.ry, c->parts.px, c->parts.py}, c->n, c->s, c->bsPart.p,
                       c->bsLim.p, raw);
         bs::parameters(raw, out);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_field_energy(csrst_ctx *c, double *out)
+{
+    try {
+        if (!c || !c->ready || !out) throw Error(CSRST_ESTATE, "not initialised");
+        // global grid + window - the global nodes under the window
+        const int Nx = c->p.Nx, Ny = c->p.Ny;
+        const int s0 = c->ddim[0][0], s1 = c->ddim[1][0];
+        const int lo0 = std::max(0, s0), lo1 = std::max(0, s1);
+        const int hi0 = std::min(Nx - 1, s0 + c->W - 1), hi1 = std::min(Ny - 1, s1 + c->H - 1);
+        long double acc[2] = {0, 0};
+        std::vector<DD> part(kRedBlocks * 2);
+        auto sum = [&](FieldDesc E, FieldDesc H, int i0, int j0, int ni, int nj, long double sign) {
+            if (ni <= 0 || nj <= 0) return;
+            k_energy_sums<<<kRedBlocks, kBlock, 0, c->s>>>(E, H, i0, j0, ni, nj, c->partial.p);
+            HU_LAUNCH_CHECK();
+            HU_CHECK(hipMemcpyAsync(part.data(), c->partial.p, kRedBlocks * 2 * sizeof(DD), hipMemcpyDeviceToHost,
+                                    c->s));
+            HU_CHECK(hipStreamSynchronize(c->s));
+            for (int b = 0; b < kRedBlocks; ++b)
+                for (int m = 0; m < 2; ++m)
+                    acc[m] += sign * ((long double)part[b * 2 + m].hi + (long double)part[b * 2 + m].lo);
+        };
+        sum(c->fs.f[F_GE], c->fs.f[F_GH], 0, 0, Nx, Ny, 1);
+        sum(c->fs.f[F_ME], c->fs.f[F_MH], 0, 0, c->W, c->H, 1);
+        sum(c->fs.f[F_GE], c->fs.f[F_GH], lo0, lo1, hi0 - lo0 + 1, hi1 - lo1 + 1, -1);
+        const long double area = (long double)c->hr[0] * c->hr[1];
+        out[0] = (double)(acc[0] * area);
+        out[1] = (double)(acc[1] * area);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);

@@ -18,6 +18,8 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -249,6 +251,117 @@ private:
     Engine &e_;
     size_t n_;
     mutable double d_[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// PartBunchState (PartBunch.hh:290-323): what PartBunch::calcBeamParameters
+// fills (csrst_beam_stats); d = 0, 1 for x, y
+struct PartBunchState {
+    double v[CSRST_BEAM_STATS] = {};
+    double get_meanEnergy() const { return v[0]; }
+    double get_rrms(int d) const { return v[1 + d]; }
+    double get_prms(int d) const { return v[3 + d]; }
+    double get_norm_emit(int d) const { return v[5 + d]; }
+    double get_rmean(int d) const { return v[7 + d]; }
+    double get_pmean(int d) const { return v[9 + d]; }
+    double get_Dy() const { return v[11]; }
+    double get_DDy() const { return v[12]; }
+    double get_dE() const { return v[13]; }
+    double get_rmin(int d) const { return v[14 + d]; }
+    double get_rmax(int d) const { return v[16 + d]; }
+};
+
+inline void calcBeamParameters(const Engine &e, PartBunchState &state)
+{
+    check(csrst_beam_stats(e.ctx(), state.v), e.ctx());
+}
+
+// CalcEnergyCommand (CalcEnergy.cpp:60-153): total field energy in nJ, from the
+// sums over the global grid and the window (csrst_field_energy); the E term
+// averages this call's sum with the previous call's.
+class CalcEnergy {
+public:
+    explicit CalcEnergy(const Engine &e) : e_(e) {}
+    double execute()
+    {
+        const double eps0 = 8.854187817e-12, mu0 = 1.256637061e-06;   // Physics.cpp:41-51
+        double sums[2];
+        check(csrst_field_energy(e_.ctx(), sums), e_.ctx());
+        const double total = (0.25 * eps0 * (sums[0] + eOld_) + 0.5 * mu0 * sums[1]) * 1e9;
+        eOld_ = sums[0];
+        return total;
+    }
+
+private:
+    const Engine &e_;
+    double eOld_ = 0.0;
+};
+
+// DataSinkCommand (DataSink.cpp:22-148): the statistics file in SDDS, one row
+// per call, columns as the reference writes them (width 25, 15 digits,
+// scientific).  The last column is the CalcEnergy value times 1e9, as
+// DataSink.cpp:146 prints it.
+class DataSink {
+public:
+    DataSink(const Engine &e, const std::string &path) : cec_(e), out_(path)
+    {
+        if (!out_) throw Error(CSRST_EINVAL, "cannot write " + path);
+        out_.precision(15);
+        out_.setf(std::ios::scientific, std::ios::floatfield);
+        header();
+    }
+    void execute(const Engine &e, double t, PartBunchState &state)
+    {
+        calcBeamParameters(e, state);
+        const double energy = cec_.execute();
+        const int w = 25;
+        out_ << std::setw(w) << t << std::setw(w) << state.get_meanEnergy();
+        for (int d = 0; d < 2; ++d) out_ << std::setw(w) << state.get_rrms(d);
+        for (int d = 0; d < 2; ++d) out_ << std::setw(w) << state.get_prms(d);
+        for (int d = 0; d < 2; ++d) out_ << std::setw(w) << state.get_norm_emit(d);
+        for (int d = 0; d < 2; ++d) out_ << std::setw(w) << state.get_rmean(d);
+        for (int d = 0; d < 2; ++d) out_ << std::setw(w) << state.get_pmean(d);
+        out_ << std::setw(w) << state.get_Dy() << std::setw(w) << state.get_DDy() << std::setw(w) << state.get_dE()
+             << std::setw(w) << energy * 1e9 << std::endl;
+    }
+
+private:
+    void header()
+    {
+        struct Col {
+            const char *name, *units, *desc;
+        };
+        static const Col cols[] = {
+            {"t,      ", "s", "1 Time"},
+            {"energy, ", "MeV", "2 Mean Energy"},
+            {"rms_x,  ", "m ", "3 RMS Beamsize in x  "},
+            {"rms_y,  ", "m ", "4 RMS Beamsize in y  "},
+            {"rms_px, ", "1 ", "5 RMS Momenta in x  "},
+            {"rms_py, ", "1 ", "6 RMS Momenta in y  "},
+            {"emit_x, ", "m ", "7 Normalized Emittance x  "},
+            {"emit_y, ", "m ", "8 Normalized Emittance y  "},
+            {"mean_x, ", "m ", "9 Mean Beam Position in x  "},
+            {"mean_y, ", "m ", "10 Mean Beam Position in y  "},
+            {"mean_px, ", "1 ", "11 Mean Beam Momentum in x  "},
+            {"mean_py, ", "1 ", "12 Mean Beam Momentum in y  "},
+            {"Dy,     ", "m ", "13 Dispersion in y  "},
+            {"DDy,    ", "1 ", "14 Derivative of dispersion in y  "},
+            {"dE,     ", "MeV ", "15 energy spread of the beam  "},
+            {"Efld,     ", "nJ ", "16 total field energy   "},
+        };
+        out_ << "SDDS1\n"
+             << "&description text=\"Statistics data \", contents=\"stat parameters\" &end\n"
+             << "&parameter name=processors, type=long, description=\"Number of Processors\" &end\n"
+             << "&parameter name=revision, type=string, description=\"svn revision of CSRST\" &end\n";
+        for (const Col &c : cols)
+            out_ << "&column name=" << c.name << "type=double, units=" << c.units << ", description=\"" << c.desc
+                 << "\" &end\n";
+        out_ << "&data mode=ascii &end\n"
+             << "Cores used 1\n"
+             << "CSRST " << csrst_version() << std::endl;
+    }
+
+    CalcEnergy cec_;
+    std::ofstream out_;
 };
 
 class TETMSolver {

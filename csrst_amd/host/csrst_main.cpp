@@ -1,11 +1,15 @@
 // csrst -- driver of the GPU time step with the reference's command line and
 // main loop (main.cpp:76-532): Bend and fields, bunch, Poisson start unless
 // -I, then per step updateField_firstStep, updateField_secondStep, the wake
-// sample and moveMesh (main.cpp:390-506).  Prints one summary line per
-// --statmodulus steps and writes the CSR wake to --wake FILE.
+// sample and moveMesh (main.cpp:390-506).  Every --statmodulus steps prints a
+// summary line and appends the beam statistics to Data/maxwadis.stat
+// (DataSinkCommand); writes the CSR wake to --wake FILE.
+#include <sys/stat.h>
+
 #include <chrono>
 #include <cstdio>
 #include <fstream>
+#include <memory>
 
 #include "csrst.hpp"
 
@@ -52,6 +56,13 @@ int main(int argc, char **argv)
                         poisson.cycles[1], poisson.cycles[2], poisson.resid[0], poisson.resid[1], poisson.resid[2]);
         }
         csrst::TETMSolver solver(engine);
+        // DataSinkCommand writes Data/maxwadis.stat (DataSink.cpp:22-30)
+        std::unique_ptr<csrst::DataSink> sink;
+        csrst::PartBunchState state;
+        if (cmd.statmodulus > 0) {
+            ::mkdir("Data", 0755);
+            sink.reset(new csrst::DataSink(engine, "Data/maxwadis.stat"));
+        }
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = cmd.startstep; it < cmd.startstep + cmd.numsteps; ++it) {
             solver.updateField_firstStep();
@@ -63,6 +74,7 @@ int main(int argc, char **argv)
                 bunch.get_pmean(pm);
                 std::printf("step %d: rmean %.9e %.9e pmean %.9e %.9e energy %.9e\n", it + 1, rm[0], rm[1], pm[0],
                             pm[1], bunch.fieldEnergy());
+                sink->execute(engine, (it - cmd.startstep + 1) * engine.dt(), state);   // main.cpp:435-444
             }
         }
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -170,6 +170,13 @@ class TwoGridStep:
         check(_lib.lib().csrst_diag(self.h, dptr(out)), self.h)
         return out
 
+    def fieldEnergySums(self):
+        """(sum E.E, sum (Hzx+Hzy)^2) x cell area over global + window - overlap,
+        the sums of CalcEnergyCommand::execute (CalcEnergy.cpp:60-124)."""
+        out = np.zeros(2)
+        check(_lib.lib().csrst_field_energy(self.h, dptr(out)), self.h)
+        return out
+
     def calcBeamParameters(self):
         """PartBunch::calcBeamParameters (PartBunch.cpp:584-657) of the current bunch,
         as a dict keyed by csrst_amd.stats.FIELDS."""

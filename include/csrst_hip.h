@@ -110,6 +110,12 @@ int csrst_diag(csrst_ctx *c, double *out);
  * into the frame of the mean momentum; double-double sums. */
 #define CSRST_BEAM_STATS 20
 int csrst_beam_stats(csrst_ctx *c, double *out);
+/* The field sums of CalcEnergyCommand::execute (CalcEnergy.cpp:60-124) over the
+ * global grid plus the window minus the global nodes under the window, times
+ * the cell area: out[0] = sum E.E, out[1] = sum (Hzx + Hzy)^2.  The command's
+ * energy, 1/4 eps0 (out[0] + previous out[0]) + 1/2 mu0 out[1], keeps the
+ * previous call's out[0], which the caller holds. */
+int csrst_field_energy(csrst_ctx *c, double *out);
 
 /* ---- moving-window production step -------------------------------------- */
 

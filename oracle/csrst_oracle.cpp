@@ -2256,6 +2256,32 @@ void orc_sim_diag(void *sp, double *out)
 }
 
 
+/* CalcEnergyCommand::execute (CalcEnergy.cpp:60-124): sums of E.E and
+ * (Hzx+Hzy)^2 over the global grid plus the window minus the global nodes
+ * under the window, times the cell area */
+void orc_sim_field_energy(void *sp, double *out)
+{
+    Sim *s = as<Sim>(sp);
+    long double e2 = 0, h2 = 0;
+    auto add = [&](const VField &E, const VField &H, int i0, int j0, int i1, int j1, int sign) {
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const double ex = E.at(0, i, j), ey = E.at(1, i, j);
+                const double hz = H.at(0, i, j) + H.at(1, i, j);
+                e2 += sign * (long double)(ex * ex + ey * ey);
+                h2 += sign * (long double)(hz * hz);
+            }
+    };
+    add(s->E, s->H, 0, 0, s->p.Nx - 1, s->p.Ny - 1, 1);
+    add(s->mE, s->mH, 0, 0, s->W - 1, s->Hh - 1, 1);
+    const int lo0 = std::max(0, s->ddim[0][0]), lo1 = std::max(0, s->ddim[1][0]);
+    const int hi0 = std::min(s->p.Nx - 1, s->ddim[0][0] + s->W - 1), hi1 = std::min(s->p.Ny - 1, s->ddim[1][0] + s->Hh - 1);
+    if (lo0 <= hi0 && lo1 <= hi1) add(s->E, s->H, lo0, lo1, hi0, hi1, -1);
+    const long double area = (long double)s->hr[0] * s->hr[1];
+    out[0] = (double)(e2 * area);
+    out[1] = (double)(h2 * area);
+}
+
 int orc_poisson_classes(void *bp, uint8_t *flag, uint8_t *active)
 {
     std::vector<uint8_t> f, a;

@@ -93,6 +93,8 @@ int   orc_sim_get_field(void *s, int which, double *out);
 void  orc_sim_get_particles(void *s, double *R, double *P, double *oldR);
 int   orc_sim_get_wake(void *s, double *out);
 /* compensated diagnostics: rmean(2) pmean(2) qtotal field_energy */
+/* CalcEnergyCommand sums (CalcEnergy.cpp:60-124): out[2] = sum E.E, sum (Hzx+Hzy)^2, times cell area */
+void  orc_sim_field_energy(void *s, double *out);
 void  orc_sim_diag(void *s, double *out);
 /* set a field (same layout as get) */
 int   orc_sim_set_field(void *s, int which, const double *in);

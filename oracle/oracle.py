@@ -307,3 +307,10 @@ class Sim:
         out = np.zeros(6)
         lib().orc_sim_diag(C.c_void_p(self.h), _ptr(out))
         return out
+
+    def field_energy_sums(self):
+        """CalcEnergyCommand sums (CalcEnergy.cpp:60-124): sum E.E and sum
+        (Hzx+Hzy)^2 over global + window - overlap, times the cell area."""
+        out = np.zeros(2)
+        lib().orc_sim_field_energy(C.c_void_p(self.h), _ptr(out))
+        return out

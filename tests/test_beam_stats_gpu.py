@@ -51,3 +51,24 @@ def test_moving_window_bunch_parameters(gpu, fp32):
         Rm, Pm = m.particles()
         check(m.calcBeamParameters(), O.beam_parameters(Rm, Pm), "moving window fp32=%s step %d" % (fp32, 4 * it))
         m.step(4)
+
+
+def test_field_energy_sums_match_oracle(gpu):
+    """csrst_field_energy against the oracle's CalcEnergyCommand sums over the
+    global grid plus the window minus the overlap (CalcEnergy.cpp:60-124)."""
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lenBefore"], length=GEOM["lenAfter"],
+                   width=GEOM["width"], phi=GEOM["phiDeg"], Ekin=GEOM["ekinMeV"], Bz=GEOM["bz"], order=1,
+                   statmodulus=1)
+    R, P, Q = gaussian_bunch(2000, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=8)
+    g = TwoGridStep(cmd)
+    g.init_bunch(R, P, Q)
+    g.computeField()
+    o = O.Sim(1100, 120, order=1, statModulus=1, **GEOM)
+    o.init(R, P, Q)
+    o.poisson()
+    for it in range(3):
+        a, b = g.fieldEnergySums(), o.field_energy_sums()
+        print("step %d: E.E %.12e / %.12e, H^2 %.12e / %.12e" % (it, a[0], b[0], a[1], b[1]))
+        assert np.all(np.abs(a - b) <= 1e-12 * np.abs(b))
+        g.step(1)
+        o.step(1)

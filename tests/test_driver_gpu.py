@@ -1,0 +1,58 @@
+"""The C++ driver (csrst_amd/csrst) on the GPU: the statistics file of
+DataSinkCommand (Data/maxwadis.stat) has the reference's SDDS header and one
+row per --statmodulus steps whose values equal what the library returns for
+the same run (PartBunch::calcBeamParameters, CalcEnergyCommand)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from csrst_amd import Commands, TwoGridStep, gaussian_bunch
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "csrst_amd", "csrst")
+GEOM = dict(lengthbefore=0.05, length=0.01, width=0.0128, phi=10.0, Ekin=50.59, Bz=0.1704)
+EPS0, MU0 = 8.854187817e-12, 1.256637061e-06   # Physics.cpp:41-51
+
+
+def test_driver_statistics_file(gpu, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("driver not built")
+    R, P, Q = gaussian_bunch(3000, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=12, energy_spread=1e-3)
+    np.column_stack([R, P, Q]).astype(np.float64).tofile(str(tmp_path / "bunch.bin"))
+    steps, stat = 6, 3
+    args = [EXE, "--numgridpx", "1100", "--numgridpy", "120", "--order", "1", "-I", "--numsteps", str(steps),
+            "--statmodulus", str(stat), "--bunch", "bunch.bin"]
+    for k, v in GEOM.items():
+        args += ["--" + k, repr(v)]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(tmp_path / "Data" / "maxwadis.stat").read().splitlines()
+    header = open(os.path.join(ROOT, "tests", "golden", "maxwadis_header.txt")).read().splitlines()
+    assert lines[:len(header)] == header
+    assert lines[len(header)].startswith("CSRST ")
+    rows = [np.array([float(x) for x in l.split()]) for l in lines[len(header) + 1:]]
+    assert len(rows) == steps // stat and all(len(row) == 16 for row in rows)
+    # the same run through the library
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lengthbefore"], length=GEOM["length"],
+                   width=GEOM["width"], phi=GEOM["phi"], Ekin=GEOM["Ekin"], Bz=GEOM["Bz"], order=1, statmodulus=stat)
+    g = TwoGridStep(cmd)
+    g.init_bunch(R, P, Q)
+    dt = g.info()["dt"]
+    e_old = 0.0
+    for k, row in enumerate(rows):
+        g.step(stat)
+        s = g.calcBeamParameters()
+        sums = g.fieldEnergySums()
+        energy = (0.25 * EPS0 * (sums[0] + e_old) + 0.5 * MU0 * sums[1]) * 1e9
+        e_old = sums[0]
+        want = np.array([(k + 1) * stat * dt, s["Ekin"], s["rrms_x"], s["rrms_y"], s["prms_x"], s["prms_y"],
+                         s["emit_x"], s["emit_y"], s["rmean_x"], s["rmean_y"], s["pmean_x"], s["pmean_y"], s["Dy"],
+                         s["DDy"], s["dE"], energy * 1e9])
+        print("row %d: %s" % (k, " ".join("%.6e" % v for v in row)))
+        assert energy > 0
+        np.testing.assert_allclose(row, want, rtol=2e-14, atol=0)
