@@ -1252,7 +1252,7 @@ __device__ inline DD dd_add(DD x, DD y)
     const double hi = s + e;
     return DD{hi, e - (hi - s)};
 }
-constexpr int kRedBlocks = 240;
+constexpr int kRedBlocks = 1024;   // blocks of the particle reductions (partials of 4 DD each)
 
 template <class T>
 __global__ void k_moments(Parts<T> p, Geo g, const WinState *ws, DD *partial)
@@ -1278,14 +1278,32 @@ __global__ void k_moments(Parts<T> p, Geo g, const WinState *ws, DD *partial)
         for (int m = 0; m < 4; ++m) partial[blockIdx.x * 4 + m] = sh[m][0];
 }
 
-__global__ void k_moments_final(const DD *partial, int n, double *mom)
+// M double-double partials per block, kRedBlocks blocks: one workgroup of kB
+// threads sums them in a fixed order (strided per thread, then a tree)
+template <int M>
+__device__ void reduce_partials(const DD *partial, DD (&out)[M])
 {
-    if (threadIdx.x || blockIdx.x) return;
-    for (int m = 0; m < 4; ++m) {
+    __shared__ DD sh[M][kB];
+    for (int m = 0; m < M; ++m) {
         DD s{0, 0};
-        for (int b = 0; b < kRedBlocks; ++b) s = dd_add(s, partial[b * 4 + m]);
-        mom[m] = (s.hi + s.lo) / n;
+        for (int b = threadIdx.x; b < kRedBlocks; b += kB) s = dd_add(s, partial[b * M + m]);
+        sh[m][threadIdx.x] = s;
     }
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int m = 0; m < M; ++m) sh[m][threadIdx.x] = dd_add(sh[m][threadIdx.x], sh[m][threadIdx.x + w]);
+        __syncthreads();
+    }
+    for (int m = 0; m < M; ++m) out[m] = sh[m][0];
+}
+
+__global__ void __launch_bounds__(kB) k_moments_final(const DD *partial, int n, double *mom)
+{
+    DD s[4];
+    reduce_partials<4>(partial, s);
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 4; ++m) mom[m] = (s[m].hi + s[m].lo) / n;
 }
 
 // Wake: E along the line through the centroid in the direction of the mean
@@ -1343,13 +1361,13 @@ __global__ void k_dp_partial(Parts<T> p, DD *partial)
 }
 
 template <class T>
-__global__ void k_dp_final(const DD *partial, int n, WinState *ws, double *shift)
+__global__ void __launch_bounds__(kB) k_dp_final(const DD *partial, int n, WinState *ws, double *shift)
 {
-    if (threadIdx.x || blockIdx.x) return;
+    DD s[2];
+    reduce_partials<2>(partial, s);
+    if (threadIdx.x) return;
     for (int m = 0; m < 2; ++m) {
-        DD s{0, 0};
-        for (int b = 0; b < kRedBlocks; ++b) s = dd_add(s, partial[b * 2 + m]);
-        const double mean = (s.hi + s.lo) / n;
+        const double mean = (s[m].hi + s[m].lo) / n;
         const T pn = (T)(ws->p0[m] + mean);
         shift[m] = (double)pn - ws->p0[m];
         ws->p0[m] = (double)pn;
@@ -1555,7 +1573,7 @@ void sort_particles(csrst_mw *m)
     std::swap(m->partBuf.p, m->partBuf2.p);
     std::swap(m->partBuf.n, m->partBuf2.n);
     mw::k_dp_partial<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->partial.p);
-    mw::k_dp_final<T><<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->ws.p, m->shift.p);
+    mw::k_dp_final<T><<<1, mw::kB, 0, m->s>>>(m->partial.p, m->n, m->ws.p, m->shift.p);
     mw::k_dp_apply<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->shift.p);
     HU_LAUNCH_CHECK();
 }
@@ -1564,7 +1582,7 @@ template <class T>
 void wake(csrst_mw *m)
 {
     mw::k_moments<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->geo, m->ws.p, m->partial.p);
-    mw::k_moments_final<<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
+    mw::k_moments_final<<<1, mw::kB, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
     mw::k_wake<T><<<(m->nsWake + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(m->fields<T>(), m->geo, m->ws.p, m->mom.p,
                                                                           m->nsWake, m->wacc.p);
     HU_LAUNCH_CHECK();
@@ -1640,7 +1658,7 @@ void mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid)
     const int W = m->W, H = m->H;
     const size_t nn = (size_t)W * H;
     mw::k_moments<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->geo, m->ws.p, m->partial.p);
-    mw::k_moments_final<<<1, 1, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
+    mw::k_moments_final<<<1, mw::kB, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
     double mom[4];
     HU_CHECK(hipMemcpyAsync(mom, m->mom.p, sizeof(mom), hipMemcpyDeviceToHost, m->s));
     HU_CHECK(hipStreamSynchronize(m->s));
