@@ -112,18 +112,22 @@ def test_bunch_advances_at_c(gpu):
     assert abs(adv / (30 * 299792458.0 * dt) - 1.0) < 1e-3
 
 
-@pytest.mark.parametrize("order,fp32,tol", [(0, False, 1e-12), (1, False, 1e-12), (2, False, 1e-12),
-                                            (1, True, 2e-6), (2, True, 2e-6)])
-def test_first_step_ex_is_deposit(gpu, order, fp32, tol):
+@pytest.mark.parametrize("order,fp32,tol,angle", [(0, False, 1e-12, 0.0), (1, False, 1e-12, 0.0),
+                                                  (2, False, 1e-12, 0.0), (1, True, 2e-6, 0.0), (2, True, 2e-6, 0.0),
+                                                  (1, False, 1e-12, 25.0), (2, False, 1e-12, 25.0),
+                                                  (2, True, 2e-6, 25.0)])
+def test_first_step_ex_is_deposit(gpu, order, fp32, tol, angle):
     """From zero fields, after one step Ex = -c dt Z0 Jx exactly (the first
     half step of H is zero, the second does not feed back into Ex), with Jx
-    the oracle deposit of the first push R0 -> R0 + 2 (0.5 c dt) P/gamma."""
+    the oracle deposit of the first push R0 -> R0 + 2 (0.5 c dt) P/gamma.
+    At 25 degrees the particles also cross cell rows: three segments."""
     b = O.Bend(Nx=NX, Ny=NY, **GEOM)
     hx, hy = b.hr
     n = 3000
     rng = np.random.default_rng(5)
     R = np.stack([0.02 + rng.normal(0, 0.3e-3, n), 0.0064 + rng.normal(0, 0.4e-3, n)], 1)
-    P = np.stack([np.full(n, 99.995), np.zeros(n)], 1)
+    th = np.radians(angle)
+    P = np.stack([np.full(n, 99.995 * np.cos(th)), np.full(n, 99.995 * np.sin(th))], 1)
     Q = np.full(n, -1e-22)
     m = MovingWindowStep(256, 64, hx, hy, order=order, fp32=fp32, sortEvery=0, statModulus=0, **GEOM)
     m.init_bunch(R, P, Q)
@@ -148,7 +152,7 @@ def test_first_step_ex_is_deposit(gpu, order, fp32, tol):
     scale = np.abs(expect).max()
     d = np.abs(Ex - expect)
     j, i = np.unravel_index(np.argmax(d), d.shape)
-    print("order %d fp32 %s: max |Ex - expect| / max |expect| = %.2e" % (order, fp32, d.max() / scale))
+    print("order %d fp32 %s angle %g: max |Ex - expect| / max |expect| = %.2e" % (order, fp32, angle, d.max() / scale))
     assert d.max() < tol * scale, "max |Ex - expect| = %.3e of %.3e at (%d, %d): %r vs %r" % (
         d.max(), scale, i, j, Ex[j, i], expect[j, i])
 
