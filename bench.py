@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--particles", type=int, default=0, help="override the particle count")
+    ap.add_argument("--angle", type=float, default=0.0,
+                    help="bunch direction in degrees from the beam axis (a bend's later part); not the headline")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
@@ -140,7 +142,7 @@ def main():
     n = a.particles or cfg["n"]
     print("bench: config %s, %d particles, %dx%d cells, %d+%d steps" % (a.config, n, cfg["W"], cfg["H"], a.warmup,
                                                                         a.steps), file=sys.stderr, flush=True)
-    R, P, Q = bunch_for(cfg, n=n, seed=1)
+    R, P, Q = bunch_for(cfg, n=n, seed=1, angle_deg=a.angle)
     t32, tim32, wake32, info32 = run_mw(cfg, True, a.steps, a.warmup, R, P, Q, a.stat)
     print("bench: fp32 %.3f s, timing %s" % (t32, tim32), file=sys.stderr, flush=True)
     cells = cfg["W"] * cfg["H"]
@@ -163,6 +165,8 @@ def main():
     workload = ("%s: %dx%d window dx=%g um, %.0f deg bend R=%g m, gamma %g, %g nC, %d particles, shape S%d, "
                 "sort every %d" % (a.config, cfg["W"], cfg["H"], cfg["dx"] * 1e6, cfg["phiDeg"], cfg["R"],
                                    cfg["gamma"], cfg["charge_nC"], n, cfg["order"], cfg["sortEvery"]))
+    if a.angle:
+        workload += ", bunch at %g deg" % a.angle
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()

@@ -48,10 +48,13 @@ def config(name):
     raise KeyError(name)
 
 
-def bunch_for(cfg, n=None, seed=1, s_offset=-5e-3):
-    """Seeded bunch of the configuration, centred s_offset before the bend entry, mid-chamber."""
+def bunch_for(cfg, n=None, seed=1, s_offset=-5e-3, angle_deg=0.0):
+    """Seeded bunch of the configuration, centred s_offset before the bend entry, mid-chamber,
+    moving at angle_deg to the beam axis (0: along it)."""
     from .bunch import gaussian_bunch
     n = cfg["n"] if n is None else n
     center = (cfg["lenBefore"] + s_offset, 0.5 * cfg["width"])
+    th = np.radians(angle_deg)
     return gaussian_bunch(n, center, cfg["sigma_s"], cfg["sigma_x"], cfg["gamma"], cfg["charge_nC"], seed=seed,
-                          energy_spread=cfg["espread"], chirp=cfg["chirp"], modulation=cfg.get("modulation"))
+                          direction=(float(np.cos(th)), float(np.sin(th))), energy_spread=cfg["espread"],
+                          chirp=cfg["chirp"], modulation=cfg.get("modulation"))

@@ -511,15 +511,19 @@ __device__ __forceinline__ void rotate_in(T (&o)[kRun], T v)
 // Push of one workgroup chunk (kChunk sorted particles): thread t owns
 // particles base + kPPT t .. + kPPT - 1, so its consecutive particles share a
 // cell and their segment currents share stencil anchors.  Currents collect in
-// two register slots per lane (one anchor each); a segment whose anchor
-// matches neither (a run crossing a cell boundary) is added by its lane into
-// the chunk's LDS tile, and the slots at the end are reduced across the
-// wavefront once per distinct anchor (DPP sums).  The tile is flushed with one
-// global atomic per touched node.  Loops stay rolled where unrolling would
+// two register slots per lane (one anchor each).  A segment whose anchor
+// matches neither (a run crossing a cell boundary, or a bunch moving at an
+// angle, whose runs have up to four anchors) goes into the chunk's LDS tile;
+// at the end the slots are reduced across the wavefront once per distinct
+// anchor into the tile.  The tile is flushed with one global atomic per
+// touched node.  Loops stay rolled where unrolling would
 // not pay: the kernel has to fit the instruction cache.
 // At most three waves per SIMD: at four (the register count allows it for
 // FP32 S2) the push runs 10 % slower, at two 27 % slower.
-template <class T, int ORDER, int COMP>
+// MISSCOOP: slot misses through the wavefront (a tilted bunch: its misses are
+// many and share anchors) instead of per-lane LDS atomics (an axial bunch:
+// misses are rare, and the cooperative path in the loop costs 20 %).
+template <class T, int ORDER, int COMP, bool MISSCOOP>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3))) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
     __shared__ T acc[kTH * kTW];
@@ -559,8 +563,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
             else atomic_add_real(&J[(size_t)ringy(j, ry, f.Hp) * f.Wp + ringx(i, rx, f.Wp)], v);
         }
     };
-    // per-lane add of one stencil (LDS tile, global atomics off the tile):
-    // for the rare segment whose anchor matches neither register slot
+    // per-lane add of one stencil (LDS tile, global atomics off the tile)
     auto lane_add = [&](int I0, int J0, const T *w) {
 #pragma unroll
         for (int t = 0; t < NN; ++t) node_add(I0 + (t % NX), J0 + (t / NX), w[t]);
@@ -736,7 +739,9 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
                     }
                     sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
                 }
-                // register accumulation by anchor
+                // register accumulation by anchor; a segment matching neither
+                // slot goes to the LDS tile (MISSCOOP: through the wavefront)
+                bool miss = false;
                 if (act) {
                     if (hasA && sc.I0 == aI && sc.J0 == aJ) {
 #pragma unroll
@@ -752,10 +757,13 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
                         hasB = true; bI = sc.I0; bJ = sc.J0;
 #pragma unroll
                         for (int t = 0; t < NN; ++t) accB[t] = sc.w[t];
+                    } else if (MISSCOOP) {
+                        miss = true;
                     } else {
                         lane_add(sc.I0, sc.J0, sc.w);
                     }
                 }
+                if (MISSCOOP && __any(miss)) coop_add(miss, sc.I0, sc.J0, sc.w);
             }
             // ---- drift back by half a step (TETMSolver.hh:210, 265)
             if (live) {
@@ -1455,6 +1463,10 @@ struct csrst_mw {
 
     int cur = 0;   // which of the Ex/Hz (and PML psi) ping-pong buffers is current
     int npml = 0;  // PML columns at each window end
+    // the bunch's reference momentum points more than kTiltDeg off the x axis
+    // (set at set-up and after each batch of steps from the device's value):
+    // the push sends slot misses through the wavefront
+    bool tilted = false;
     DBuf<unsigned char> pmlBuf;   // psih, psih', psie planes + [4][W] coefficients
 
     template <class T>
@@ -1546,8 +1558,13 @@ void launch_push(csrst_mw *m)
 {
     mw::PushConst<T> c;
     push_consts(m, c);
-    hipLaunchKernelGGL((mw::k_push<T, ORDER, COMP>), dim3((m->n + mw::kChunk - 1) / mw::kChunk), dim3(mw::kB), 0, m->s,
-                       m->parts<T>(m->partBuf), m->fields<T>(), m->geo, c, m->ws.p);
+    const dim3 grid((m->n + mw::kChunk - 1) / mw::kChunk);
+    if (m->tilted)
+        hipLaunchKernelGGL((mw::k_push<T, ORDER, COMP, true>), grid, dim3(mw::kB), 0, m->s, m->parts<T>(m->partBuf),
+                           m->fields<T>(), m->geo, c, m->ws.p);
+    else
+        hipLaunchKernelGGL((mw::k_push<T, ORDER, COMP, false>), grid, dim3(mw::kB), 0, m->s, m->parts<T>(m->partBuf),
+                           m->fields<T>(), m->geo, c, m->ws.p);
 }
 
 template <class T, int COMP>
@@ -1632,11 +1649,20 @@ void one_step(csrst_mw *m, bool timed)
     ++m->steps;
 }
 
+// measured crossover of the two push forms at c2 (FP32): equal near 15 deg;
+// at 30 deg the cooperative form is 20 % faster, along x 22 % slower
+constexpr double kTiltDeg = 15.0;
+bool tilted(const double p0[2])
+{
+    return std::fabs(p0[1]) > std::tan(kTiltDeg * M_PI / 180.0) * std::fabs(p0[0]);
+}
+
 void check_state(csrst_mw *m)
 {
     mw::WinState h;
     HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
     HU_CHECK(hipStreamSynchronize(m->s));
+    m->tilted = tilted(h.p0);
     if (h.err & 1) throw Error(CSRST_EWINDOW, "particle stencil left the window");
     if (h.err & 2) throw Error(CSRST_EWINDOW, "particle crossed too many cell lines in one push");
 }
@@ -1871,6 +1897,7 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
         HU_CHECK(hipMemcpy(m->partBuf.p, host.data(), host.size(), hipMemcpyHostToDevice));
         mw::k_state_init<<<1, 1, 0, m->s>>>(m->ws.p, wx0, wy0);
         mw::k_set_p0<<<1, 1, 0, m->s>>>(m->ws.p, p0[0], p0[1]);
+        m->tilted = tilted(p0);
         const dim3 gg((m->W + 2 * mw::G + mw::kB - 1) / mw::kB, m->H + 2 * mw::G);
         if (f32) hipLaunchKernelGGL((mw::k_geom_all<float>), gg, dim3(mw::kB), 0, m->s, m->fields<float>(), m->geo, m->ws.p);
         else hipLaunchKernelGGL((mw::k_geom_all<double>), gg, dim3(mw::kB), 0, m->s, m->fields<double>(), m->geo, m->ws.p);
