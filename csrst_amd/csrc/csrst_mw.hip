@@ -12,12 +12,14 @@
 //            lambdaN1) read only at cut cells.  New columns/rows get their
 //            geometry from the analytic bend on the device (FP64).
 //   particles int32 global cell index + real offset in [0, 1) per axis
-//            (split precision), real momentum (beta*gamma); uniform charge.
+//            (split precision), momentum (beta*gamma) as the deviation from a
+//            reference momentum p0 held in FP64; uniform charge.  Arrays are
+//            padded to 64 elements and cell-sorted every few steps.
 //
 // Kernels per step
 //   k_push<comp=0>  drift, gather, kick+rotation, drift, S0/S1/S2 deposit of
-//                   Jx (wave-uniform stencils reduced across the wavefront,
-//                   one atomic per node), drift back
+//                   Jx (register slots per stencil anchor, wavefront sums
+//                   into an LDS tile, one global atomic per node), drift back
 //   k_heh           H half-step, Ex, H half-step in one row-marching pass
 //   k_push<comp=1>  same, depositing Jy
 //   k_ey            Ey
