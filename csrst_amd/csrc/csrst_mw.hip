@@ -602,6 +602,41 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
             }
         }
     };
+    // current of one segment xi -> xf (cell units relative to the corner of
+    // cell (cgx, cgy)), tau its fraction of the push, dlt the whole push's
+    // displacement
+    auto seg_current = [&](pm::SegCurrent<T> &sc, vec2<T> xi, vec2<T> xf, T tau, vec2<T> dlt, int cgx, int cgy) {
+        if (ORDER == 0) {
+            pm::seg_s0(sc, COMP, xi, xf, c.q, odt0);
+        } else {
+            const vec2<T> cc = pm::mk(T(0.5) * (xi.x + xf.x), T(0.5) * (xi.y + xf.y));
+            const int I = ORDER == 1 ? (int)pm::floor_(cc.x + T(0.5)) : (int)pm::floor_(cc.x);
+            const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
+            const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
+            const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
+            if (ORDER == 1) seg_s1_sep(sc, COMP, tau, a, b, qodt, dlt);
+            else seg_s2_gauss(sc, COMP, tau, a, b, qodt, dlt);
+            sc.I0 = I - 1; sc.J0 = Jn - 1;
+        }
+        sc.I0 += cgx; sc.J0 += cgy;   // global node of the first stencil entry
+    };
+    // per-wave queue of third segments: xi, xf, tau, displacement; old cell
+    __shared__ T qvAll[kB / 64][7][64];
+    __shared__ int qgAll[kB / 64][2][64];
+    T (*qv)[64] = qvAll[tid >> 6];
+    int (*qg)[64] = qgAll[tid >> 6];
+    int qn = 0;   // wave-uniform queue length
+    auto drain_queue = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        const bool mine = lane < qn;
+        pm::SegCurrent<T> sc;
+        const int k = mine ? lane : 0;
+        seg_current(sc, pm::mk(qv[0][k], qv[1][k]), pm::mk(qv[2][k], qv[3][k]), qv[4][k], pm::mk(qv[5][k], qv[6][k]),
+                    qg[0][k], qg[1][k]);
+        __builtin_amdgcn_wave_barrier();
+        coop_add(mine, sc.I0, sc.J0, sc.w);
+        qn = 0;
+    };
     // two register accumulation slots per lane
     bool hasA = false, hasB = false;
     int aI = 0, aJ = 0, bI = 0, bJ = 0;
@@ -715,31 +750,36 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
             if (live && !oneCross) err |= 2;
             const vec2<T> delta = pm::mk(e.x - o.x, e.y - o.y);
             const int nseg = live ? xs.n - 1 : 0;
+            // third segments (a push crossing a cell column and a cell row) are
+            // queued and computed 64 at a time: in-line, one lane with three
+            // segments makes its whole wavefront run a third segment pass
+            const bool has3 = nseg == 3;
+            const unsigned long long m3 = __ballot(has3);
+            if (m3) {
+                const int cnt = __popcll(m3);
+                if (qn + cnt > 64) drain_queue();
+                if (has3) {
+                    const int at = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m3 >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m3, 0u));
+                    qv[0][at] = xs.s[2].x; qv[1][at] = xs.s[2].y;
+                    qv[2][at] = xs.s[3].x; qv[3][at] = xs.s[3].y;
+                    qv[4][at] = xs.f[3] - xs.f[2];
+                    qv[5][at] = delta.x; qv[6][at] = delta.y;
+                    qg[0][at] = gx0; qg[1][at] = gy0;
+                }
+                qn += cnt;
+            }
 #pragma unroll 1
-            for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < 2; ++s) {
                 const bool act = s < nseg;
                 if (!__any(act)) break;
                 pm::SegCurrent<T> sc;
                 {
                     // slot selects instead of xs.s[s] keep the list in registers
-                    const vec2<T> xi = sel(s == 0, xs.s[0], sel(s == 1, xs.s[1], xs.s[2]));
-                    const vec2<T> xf = sel(s == 0, xs.s[1], sel(s == 1, xs.s[2], xs.s[3]));
-                    const T fi = s == 0 ? xs.f[0] : (s == 1 ? xs.f[1] : xs.f[2]);
-                    const T ff = s == 0 ? xs.f[1] : (s == 1 ? xs.f[2] : xs.f[3]);
-                    if (ORDER == 0) {
-                        pm::seg_s0(sc, COMP, xi, xf, c.q, odt0);
-                    } else {
-                        const T tau = ff - fi;
-                        const vec2<T> cc = pm::mk(T(0.5) * (xi.x + xf.x), T(0.5) * (xi.y + xf.y));
-                        const int I = ORDER == 1 ? (int)pm::floor_(cc.x + T(0.5)) : (int)pm::floor_(cc.x);
-                        const int Jn = ORDER == 1 ? (int)pm::floor_(cc.y + T(0.5)) : (int)pm::floor_(cc.y);
-                        const vec2<T> a = pm::mk((T)I + T(0.5) - xi.x, (T)Jn + T(0.5) - xi.y);
-                        const vec2<T> b = pm::mk((T)I - xi.x, (T)Jn - xi.y);
-                        if (ORDER == 1) seg_s1_sep(sc, COMP, tau, a, b, qodt, delta);
-                        else seg_s2_gauss(sc, COMP, tau, a, b, qodt, delta);
-                        sc.I0 = I - 1; sc.J0 = Jn - 1;
-                    }
-                    sc.I0 += gx0; sc.J0 += gy0;   // global node of the first stencil entry
+                    const vec2<T> xi = sel(s == 0, xs.s[0], xs.s[1]);
+                    const vec2<T> xf = sel(s == 0, xs.s[1], xs.s[2]);
+                    const T tau = s == 0 ? xs.f[1] - xs.f[0] : xs.f[2] - xs.f[1];
+                    seg_current(sc, xi, xf, tau, delta, gx0, gy0);
                 }
                 // register accumulation by anchor; a segment matching neither
                 // slot goes to the LDS tile (MISSCOOP: through the wavefront)
@@ -785,7 +825,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
         store_run(p.px, k0, p.n, PX);
         store_run(p.py, k0, p.n, PY);
     }
-    // ---- flush the register slots
+    // ---- queued third segments, then the register slots
+    if (qn) drain_queue();
 #pragma unroll 1
     for (int slot = 0; slot < 2; ++slot) {
         T w[NN];
