@@ -1548,6 +1548,42 @@ int csrst_move_window(csrst_ctx *c, int *moveI, int *moveJ)
     }
 }
 
+// copyFields (main.cpp:534-548) with Communicator::copy (Communicator.hh:212-268):
+// the window cells (i, j) whose global index (ddim0 + i, ddim1 + j) lies in the
+// global domain overwrite the global E and both H planes, and J when
+// copyCurrent is set; guard layers are not copied.  Strided device copies.
+int csrst_copy_fields(csrst_ctx *c, int copyCurrent)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "not initialised");
+        const int Nx = c->p.Nx, Ny = c->p.Ny, W = c->W, H = c->H;
+        const int ox = c->ddim[0][0], oy = c->ddim[1][0];
+        const int i0 = std::max(0, -ox), i1 = std::min(W - 1, Nx - 1 - ox);
+        const int j0 = std::max(0, -oy), j1 = std::min(H - 1, Ny - 1 - oy);
+        if (i0 <= i1 && j0 <= j1) {
+            const size_t gp = (size_t)Nx + 2, wp = (size_t)W + 2;
+            const size_t gplane = gp * ((size_t)Ny + 2), wplane = wp * ((size_t)H + 2);
+            auto copy = [&](DBuf<double> &g, const DBuf<double> &w) {
+                if (g.n != 2 * gplane || w.n != 2 * wplane) throw Error(CSRST_ESTATE, "field sizes");
+                for (int k = 0; k < 2; ++k) {
+                    double *dst = g.p + k * gplane + (size_t)(oy + j0 + 1) * gp + (size_t)(ox + i0 + 1);
+                    const double *src = w.p + k * wplane + (size_t)(j0 + 1) * wp + (size_t)(i0 + 1);
+                    HU_CHECK(hipMemcpy2DAsync(dst, gp * sizeof(double), src, wp * sizeof(double),
+                                              (size_t)(i1 - i0 + 1) * sizeof(double), (size_t)(j1 - j0 + 1),
+                                              hipMemcpyDeviceToDevice, c->s));
+                }
+            };
+            copy(c->gE, c->mEb);
+            copy(c->gH, c->mHb);
+            if (copyCurrent) copy(c->gJ, c->mJb);
+        }
+        HU_CHECK(hipStreamSynchronize(c->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
 static DBuf<double> *field_buf(csrst_ctx *c, int which, int &nn, int &mm)
 {
     switch (which) {

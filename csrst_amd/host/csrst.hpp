@@ -161,6 +161,8 @@ public:
         check(csrst_move_window(ctx_, &mi, &mj), ctx_);
         return {mi, mj};
     }
+    // copyFields (main.cpp:534-548): window E, H (and J) into the global fields
+    void copyFields(bool copyCurrent) { check(csrst_copy_fields(ctx_, copyCurrent ? 1 : 0), ctx_); }
     // IntegrateElongCommand accumulator (IntegrateElong.cpp:38-125)
     std::vector<double> wake() const
     {

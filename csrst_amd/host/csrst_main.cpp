@@ -1,9 +1,13 @@
 // csrst -- driver of the GPU time step with the reference's command line and
 // main loop (main.cpp:76-532): Bend and fields, bunch, Poisson start unless
-// -I, then per step updateField_firstStep, updateField_secondStep, the wake
-// sample and moveMesh (main.cpp:390-506).  Every --statmodulus steps prints a
-// summary line and appends the beam statistics to Data/maxwadis.stat
-// (DataSinkCommand); writes the CSR wake to --wake FILE.
+// -I, then per step updateField_firstStep, updateField_secondStep, the field
+// plots, the statistics, the wake sample and moveMesh (main.cpp:390-506).
+// Field plots (PlotBinaryVTKCommand): the global fields as Data/c_NNNN.pvtr
+// and the window as Data/mm_NNNN.pvtr at the start and every --plotmodulus
+// steps, after copyFields; Data/c_collection.pvd and mm_collection.pvd at the
+// end.  Every --statmodulus steps prints a summary line and appends the beam
+// statistics to Data/maxwadis.stat (DataSinkCommand).  Writes the CSR wake to
+// --wake FILE.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -12,6 +16,7 @@
 #include <memory>
 
 #include "csrst.hpp"
+#include "vtk.hpp"
 
 namespace {
 
@@ -32,6 +37,35 @@ csrst::Bunch read_bunch(const std::string &path)
         b.Q[k] = v[5 * k + 4];
     }
     return b;
+}
+
+// One VTK frame from planar [2][m+2][n+2] field downloads (guards included).
+csrst::vtk::Frame frame(const std::vector<double> &E, const std::vector<double> &H, const std::vector<double> &J,
+                        int n, int m, const double *hr, double ox, double oy)
+{
+    const size_t plane = (size_t)(n + 2) * (m + 2);
+    if (E.size() != 2 * plane || H.size() != 2 * plane || J.size() != 2 * plane)
+        throw csrst::Error(CSRST_EINVAL, "field download size");
+    csrst::vtk::Frame f;
+    f.nx = n; f.ny = m;
+    f.hx = hr[0]; f.hy = hr[1]; f.ox = ox; f.oy = oy;
+    for (int k = 0; k < 2; ++k) {
+        f.E[k] = {E.data() + k * plane, (size_t)n + 2};
+        f.H[k] = {H.data() + k * plane, (size_t)n + 2};
+        f.J[k] = {J.data() + k * plane, (size_t)n + 2};
+    }
+    return f;
+}
+
+// pavc.execute and mmpavc.execute (main.cpp:358-366, 416-423)
+void plot(const csrst::Engine &engine, csrst::vtk::Plot &global, csrst::vtk::Plot &window, double t)
+{
+    const csrst::Commands &cmd = engine.commands();
+    const auto inf = engine.info();
+    global.execute(t, frame(engine.field(0), engine.field(1), engine.field(5), cmd.numgridpx, cmd.numgridpy,
+                            engine.hr(), engine.origin()[0], engine.origin()[1]));
+    window.execute(t, frame(engine.field(2), engine.field(3), engine.field(4), (int)inf[5], (int)inf[6], engine.hr(),
+                            inf[1], inf[2]));
 }
 
 }  // namespace
@@ -56,27 +90,41 @@ int main(int argc, char **argv)
                         poisson.cycles[1], poisson.cycles[2], poisson.resid[0], poisson.resid[1], poisson.resid[2]);
         }
         csrst::TETMSolver solver(engine);
+        ::mkdir("Data", 0755);
         // DataSinkCommand writes Data/maxwadis.stat (DataSink.cpp:22-30)
         std::unique_ptr<csrst::DataSink> sink;
         csrst::PartBunchState state;
-        if (cmd.statmodulus > 0) {
-            ::mkdir("Data", 0755);
-            sink.reset(new csrst::DataSink(engine, "Data/maxwadis.stat"));
-        }
+        if (cmd.statmodulus > 0) sink.reset(new csrst::DataSink(engine, "Data/maxwadis.stat"));
+        // the start state is plotted without a copy (main.cpp:358-366)
+        csrst::vtk::Plot pavc("Data", "c"), mmpavc("Data", "mm");
+        plot(engine, pavc, mmpavc, 0.0);
+        if (cmd.linemodulus > 0 && cmd.linemodulus <= cmd.numsteps)
+            std::printf("csrst: --linemodulus: the field copy is done, the SaveEMFields line files are not written\n");
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = cmd.startstep; it < cmd.startstep + cmd.numsteps; ++it) {
             solver.updateField_firstStep();
             solver.updateField_secondStep();
-            engine.moveMesh();
-            if (cmd.statmodulus > 0 && it % cmd.statmodulus == cmd.statmodulus - 1) {
+            const double t = (it - cmd.startstep + 1) * engine.dt();   // currentSimulatedTime + dt
+            const bool plotStep = cmd.plotmodulus > 0 && it % cmd.plotmodulus == cmd.plotmodulus - 1;
+            if (plotStep) {   // main.cpp:416-423
+                engine.copyFields(true);
+                plot(engine, pavc, mmpavc, t);
+            }
+            if (cmd.statmodulus > 0 && it % cmd.statmodulus == cmd.statmodulus - 1) {   // main.cpp:435-444
                 double rm[2], pm[2];
                 bunch.get_rmean(rm);
                 bunch.get_pmean(pm);
                 std::printf("step %d: rmean %.9e %.9e pmean %.9e %.9e energy %.9e\n", it + 1, rm[0], rm[1], pm[0],
                             pm[1], bunch.fieldEnergy());
-                sink->execute(engine, (it - cmd.startstep + 1) * engine.dt(), state);   // main.cpp:435-444
+                sink->execute(engine, t, state);
             }
+            // main.cpp:468-476: the line dump copies E and H first
+            if (cmd.linemodulus > 0 && it % cmd.linemodulus == cmd.linemodulus - 1 && !plotStep)
+                engine.copyFields(false);
+            engine.moveMesh();   // wake sample, J = 0, moveMesh (main.cpp:445-501)
         }
+        pavc.close();   // main.cpp:521-522
+        mmpavc.close();
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const auto inf = engine.info();
         std::printf("csrst: %d steps in %.3f s, window at (%d, %d) of %d x %d\n", cmd.numsteps, sec, (int)inf[3],

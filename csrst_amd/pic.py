@@ -122,6 +122,11 @@ class TwoGridStep:
         check(_lib.lib().csrst_move_window(self.h, C.byref(mi), C.byref(mj)), self.h)
         return mi.value, mj.value
 
+    def copyFields(self, copyCurrent=True):
+        """copyFields (main.cpp:534-548): the window's E, H (and J) overwrite
+        the global fields under it; done before every field plot."""
+        check(_lib.lib().csrst_copy_fields(self.h, C.c_int(int(bool(copyCurrent)))), self.h)
+
     # -- state ------------------------------------------------------------------
     def info(self):
         a = np.zeros(9)

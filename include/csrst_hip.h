@@ -83,6 +83,12 @@ int csrst_step(csrst_ctx *c, int nsteps);
 int csrst_first_step(csrst_ctx *c);            /* TETMSolver.hh:110-211 */
 int csrst_second_step(csrst_ctx *c);           /* TETMSolver.hh:213-266 */
 int csrst_move_window(csrst_ctx *c, int *moveI, int *moveJ);   /* main.cpp:494-501 */
+/* copyFields(EFD, HFD, JFD, miniMesh E/H/J, -ddim.min, true, copyCurrent)
+ * (main.cpp:534-548, Communicator.hh:212-268): the window's E, H (and J when
+ * copyCurrent != 0) overwrite the global fields under the window, clipped to
+ * the global domain.  The reference calls it before every field plot
+ * (main.cpp:417-419) and line dump (main.cpp:472-475). */
+int csrst_copy_fields(csrst_ctx *c, int copyCurrent);
 /* One part of a step, in the order of TETMSolver.hh:110-266 and main.cpp:494-501:
  * 0 push + deposit Jx, 1 push + deposit Jy (PartBunch::push_particles),
  * 2 H half step, 3 Ex, 4 Ey (global and window sweeps), 5/6/7 exchange H/Ex/Ey

@@ -221,6 +221,26 @@ class Bend:
         return out
 
 
+def copy_window(G, Wf, pos):
+    """Communicator::copy(from=window, to=global, start=-ddim.min)
+    (Communicator.hh:212-268): window cell (i, j) lands on global cell
+    (pos[0] + i, pos[1] + j) wherever that lies in the global domain; guard
+    layers stay.  G, Wf: [2, m+2, n+2] arrays over [-1, n] x [-1, m].
+    Returns the new global array."""
+    G = np.array(G, copy=True)
+    ny, nx = G.shape[1] - 2, G.shape[2] - 2
+    h, w = Wf.shape[1] - 2, Wf.shape[2] - 2
+    for j in range(h):
+        gj = pos[1] + j
+        if not 0 <= gj < ny:
+            continue
+        for i in range(w):
+            gi = pos[0] + i
+            if 0 <= gi < nx:
+                G[:, gj + 1, gi + 1] = Wf[:, j + 1, i + 1]
+    return G
+
+
 class Sim:
     """Two-grid PIC time step of the reference (main.cpp:390-506), FP64 on the CPU."""
 
@@ -307,6 +327,13 @@ class Sim:
         out = np.zeros(6)
         lib().orc_sim_diag(C.c_void_p(self.h), _ptr(out))
         return out
+
+    def copy_fields(self, copyCurrent=True):
+        """copyFields (main.cpp:534-548) on the oracle's state: window E, H
+        (and J) into the global fields, through copy_window below."""
+        pos = self.info()["pos"]
+        for g, w in (("E", "mE"), ("H", "mH")) + ((("J", "mJ"),) if copyCurrent else ()):
+            self.set_field(g, copy_window(self.field(g), self.field(w), pos))
 
     def field_energy_sums(self):
         """CalcEnergyCommand sums (CalcEnergy.cpp:60-124): sum E.E and sum

@@ -1,7 +1,12 @@
-"""The C++ driver (csrst_amd/csrst) on the GPU: the statistics file of
-DataSinkCommand (Data/maxwadis.stat) has the reference's SDDS header and one
-row per --statmodulus steps whose values equal what the library returns for
-the same run (PartBunch::calcBeamParameters, CalcEnergyCommand)."""
+"""The C++ driver (csrst_amd/csrst) on the GPU:
+- the statistics file of DataSinkCommand (Data/maxwadis.stat) has the
+  reference's SDDS header and one row per --statmodulus steps whose values
+  equal what the library returns for the same run at the same point of the
+  step, before the window moves (PartBunch::calcBeamParameters,
+  CalcEnergyCommand);
+- the field plots (PlotBinaryVTKCommand) are byte for byte the oracle
+  writer's files of the library's fields for the same run, with copyFields
+  before every plot after the start."""
 import os
 import subprocess
 
@@ -9,6 +14,7 @@ import numpy as np
 import pytest
 
 from csrst_amd import Commands, TwoGridStep, gaussian_bunch
+from oracle import vtk as V
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +51,9 @@ def test_driver_statistics_file(gpu, tmp_path):
     dt = g.info()["dt"]
     e_old = 0.0
     for k, row in enumerate(rows):
-        g.step(stat)
+        g.step(stat - 1)
+        g.updateField_firstStep()
+        g.updateField_secondStep()
         s = g.calcBeamParameters()
         sums = g.fieldEnergySums()
         energy = (0.25 * EPS0 * (sums[0] + e_old) + 0.5 * MU0 * sums[1]) * 1e9
@@ -56,3 +64,52 @@ def test_driver_statistics_file(gpu, tmp_path):
         print("row %d: %s" % (k, " ".join("%.6e" % v for v in row)))
         assert energy > 0
         np.testing.assert_allclose(row, want, rtol=2e-14, atol=0)
+        g.moveMesh()
+
+
+def run_driver(tmp_path, R, P, Q, extra):
+    np.column_stack([R, P, Q]).astype(np.float64).tofile(str(tmp_path / "bunch.bin"))
+    args = [EXE, "--numgridpx", "1100", "--numgridpy", "120", "--order", "1", "-I", "--bunch", "bunch.bin"] + extra
+    for k, v in GEOM.items():
+        args += ["--" + k, repr(v)]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_driver_vtk_frames(gpu, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("driver not built")
+    R, P, Q = gaussian_bunch(2000, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=13, energy_spread=1e-3)
+    steps, pm = 7, 3
+    run_driver(tmp_path, R, P, Q, ["--numsteps", str(steps), "--plotmodulus", str(pm)])
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lengthbefore"], length=GEOM["length"],
+                   width=GEOM["width"], phi=GEOM["phi"], Ekin=GEOM["Ekin"], Bz=GEOM["Bz"], order=1)
+    g = TwoGridStep(cmd)
+    g.init_bunch(R, P, Q)
+    dt = g.info()["dt"]
+    want, times = {}, []
+
+    def snap(t):
+        k = len(times)
+        times.append(t)
+        want.update(V.frame_files("c_%04d" % k, g.field("E"), g.field("H"), g.field("J"), g.hr, g.origin))
+        want.update(V.frame_files("mm_%04d" % k, g.field("mE"), g.field("mH"), g.field("mJ"), g.hr,
+                                  g.info()["origin"]))
+
+    snap(0.0)
+    for it in range(steps):
+        g.updateField_firstStep()
+        g.updateField_secondStep()
+        if it % pm == pm - 1:
+            g.copyFields(True)
+            snap((it + 1) * dt)
+        g.moveMesh()
+    want["c_collection.pvd"] = V.collection_bytes("c", times)
+    want["mm_collection.pvd"] = V.collection_bytes("mm", times)
+    data = tmp_path / "Data"
+    got = sorted(n for n in os.listdir(data) if n != "maxwadis.stat")
+    assert got == sorted(want)
+    for name, b in want.items():
+        assert (data / name).read_bytes() == b, name
+    assert len(times) == 3

@@ -88,3 +88,25 @@ def test_substeps_bit_identical(gpu, order):
             for f in ("E", "H", "mE", "mH", "mJ"):
                 assert same(g.field(f), o.field(f)), "step %d, part %d: field %s differs" % (step, op, f)
     assert g.info()["pos"] == o.info()["pos"]
+
+
+def test_copy_fields_bit_identical(gpu):
+    """copyFields (main.cpp:534-548) through the C ABI against the oracle's
+    Communicator::copy restatement (window E, H and optionally J into the
+    global fields), and the steps after it, which start from the changed
+    global fields."""
+    g, o = make(1, n=1500, seed=9)
+    for _ in range(3):
+        g.step(1)
+        o.step(1)
+    for cur in (False, True):
+        before = g.field("E")
+        g.copyFields(cur)
+        o.copy_fields(cur)
+        assert not same(before, g.field("E")), "the copy left the global E unchanged"
+        compare(g, o, "copy (J %s)" % cur)
+        assert same(g.field("J"), o.field("J")), "global J after the copy"
+        for k in range(2):
+            g.step(1)
+            o.step(1)
+            compare(g, o, "step %d after copy (J %s)" % (k, cur))
