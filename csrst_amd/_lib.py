@@ -19,8 +19,10 @@ ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EWINDOW", -5: "EGEOM", -6
 SYMBOLS = [
     "csrst_version", "csrst_last_error", "csrst_device_count",
     "csrst_create", "csrst_destroy", "csrst_geometry", "csrst_boundary_cells", "csrst_init_bunch",
+    "csrst_restore_bunch",
     "csrst_step", "csrst_first_step", "csrst_second_step", "csrst_move_window", "csrst_copy_fields", "csrst_substep",
-    "csrst_download_field", "csrst_upload_field", "csrst_download_particles", "csrst_info",
+    "csrst_download_field", "csrst_upload_field", "csrst_download_particles", "csrst_download_charges",
+    "csrst_info",
     "csrst_wake", "csrst_diag", "csrst_beam_stats", "csrst_field_energy",
     "csrst_mw_create", "csrst_mw_destroy", "csrst_mw_init_bunch", "csrst_mw_step", "csrst_mw_place",
     "csrst_mw_upload_fields",

@@ -61,6 +61,7 @@ def build(verbose=True, jobs=4):
     for o in objs:
         os.remove(o)
     build_driver(verbose)
+    build_h5part(verbose)
     return LIB
 
 
@@ -72,6 +73,28 @@ def build_driver(verbose=True):
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(HERE, "..", "include"),
            os.path.join(HERE, "host", "csrst_main.cpp"), "-o", tmp, "-L" + HERE, "-lcsrst",
            "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, exe)
+    return exe
+
+
+HDF5_ROOT = "/opt/conda"
+
+
+def build_h5part(verbose=True):
+    """The H5Part converter (host/csrst_h5part.c), plain C over the HDF5 C
+    library; skipped when HDF5's headers are not installed."""
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        if verbose:
+            print("HDF5 not found under %s: csrst_h5part not built" % HDF5_ROOT, flush=True)
+        return None
+    exe = os.path.join(HERE, "csrst_h5part")
+    tmp = exe + ".tmp%d" % os.getpid()
+    lib = os.path.join(HDF5_ROOT, "lib")
+    cmd = ["gcc", "-std=gnu99", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(HDF5_ROOT, "include"),
+           os.path.join(HERE, "host", "csrst_h5part.c"), "-o", tmp, "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

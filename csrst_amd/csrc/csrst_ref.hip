@@ -1362,10 +1362,16 @@ int csrst_boundary_cells(csrst_ctx *c, uint8_t *mask, int *ij, double *data, int
     return CSRST_OK;
 }
 
-int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, int n)
+// PartBunch set-up (PartBunch.cpp:1817-1844 sort), then makeReady(dt/2)
+// (PartBunch.cpp:164-177) when oldR is null, else oldR as given (the restart
+// path of main.cpp:147-153,228, LoadH5Part::restoreBunch); the window copy
+// and exchanges of main.cpp:335-341; the step counter starts at step.
+static void init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, const double *oldR, int n,
+                       int step)
 {
-    try {
-        if (!c || !R || !P || !Q || n <= 0) throw Error(CSRST_EINVAL, "bad bunch arguments");
+    {
+        if (!R || !P || !Q || n <= 0) throw Error(CSRST_EINVAL, "bad bunch arguments");
+        if (step < 0) throw Error(CSRST_EINVAL, "negative step");
         c->n = n;
         // Distribution::getDomain (Distribution.cpp:181-207) + 5 cells, main.cpp:200-217
         double mn[2] = {R[0], R[1]}, mx[2] = {R[0], R[1]};
@@ -1417,9 +1423,13 @@ int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const doubl
             hx[k] = R[2 * s]; hy[k] = R[2 * s + 1];
             hpx[k] = P[2 * s]; hpy[k] = P[2 * s + 1];
             hq[k] = Q[s];
-            const double g = (double)pm::q_rsqrt((float)(1 + (hpx[k] * hpx[k] + hpy[k] * hpy[k])));
-            hox[k] = hx[k] - halfdt * hpx[k] * g;
-            hoy[k] = hy[k] - halfdt * hpy[k] * g;
+            if (oldR) {
+                hox[k] = oldR[2 * s]; hoy[k] = oldR[2 * s + 1];
+            } else {
+                const double g = (double)pm::q_rsqrt((float)(1 + (hpx[k] * hpx[k] + hpy[k] * hpy[k])));
+                hox[k] = hx[k] - halfdt * hpx[k] * g;
+                hoy[k] = hy[k] - halfdt * hpy[k] * g;
+            }
         }
         DBuf<double> *bufs[7] = {&c->rx, &c->ry, &c->ox, &c->oy, &c->px, &c->py, &c->q};
         std::vector<double> *hv[7] = {&hx, &hy, &hox, &hoy, &hpx, &hpy, &hq};
@@ -1442,9 +1452,29 @@ int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const doubl
         c->wacc.alloc(ns);
         c->wacc.zero(c->s);
         c->gJ.zero(c->s);
-        c->steps = 0;
+        c->steps = step;
         c->ready = true;
         HU_CHECK(hipStreamSynchronize(c->s));
+    }
+}
+
+int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, int n)
+{
+    try {
+        if (!c) throw Error(CSRST_EINVAL, "null context");
+        init_bunch(c, R, P, Q, nullptr, n, 0);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_restore_bunch(csrst_ctx *c, const double *R, const double *oldR, const double *P, const double *Q, int n,
+                        int step)
+{
+    try {
+        if (!c || !oldR) throw Error(CSRST_EINVAL, "null argument");
+        init_bunch(c, R, P, Q, oldR, n, step);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);
@@ -1647,6 +1677,18 @@ int csrst_download_particles(csrst_ctx *c, double *R, double *P, double *oldR, i
         pair(R, c->rx, c->ry);
         pair(P, c->px, c->py);
         pair(oldR, c->ox, c->oy);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_download_charges(csrst_ctx *c, double *Q, int n)
+{
+    try {
+        if (!c || !Q || n != c->n) throw Error(CSRST_EINVAL, "particle count mismatch");
+        HU_CHECK(hipStreamSynchronize(c->s));
+        HU_CHECK(hipMemcpy(Q, c->q.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);

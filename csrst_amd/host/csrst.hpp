@@ -55,6 +55,7 @@ struct Commands {
     int statmodulus = 100000;
     int linemodulus = 100000;
     int reballance = 100000;
+    int savemodulus = 100000;   // -M: restart records every M steps (main.cpp:118, commented out there)
     double phi = 0.0;
     double width = 0.10;
     double length = 0.10;
@@ -79,7 +80,21 @@ struct Commands {
     double poissontol = 1e-12;   // main.cpp:94 relativeConvergenceTolerance
     int poissonmaxit = 500;      // main.cpp:93 maxNumPoissonIterations
 
+    std::vector<std::string> given;   // long names of the arguments on the command line
+
+    // One argument: TCLAP short and long name, kind (i int, d double,
+    // s string, b switch) and the member it sets.
+    struct Opt {
+        const char *shortName, *longName;
+        char kind;
+        void *dst;
+    };
+    std::vector<Opt> options();
     static Commands parse(int argc, char **argv);
+    bool isSet(const std::string &name) const
+    {
+        return std::find(given.begin(), given.end(), name) != given.end();
+    }
     csrst_params params() const
     {
         csrst_params p;
@@ -107,6 +122,7 @@ struct Commands {
 // and momenta exactly as the reference does.
 struct Bunch {
     std::vector<double> R, P, Q;   // R, P: 2 per particle
+    std::vector<double> oldR;      // restart only (LoadH5Part::restoreBunch); empty: makeReady
     size_t size() const { return Q.size(); }
 };
 
@@ -181,6 +197,10 @@ public:
         return a;
     }
     // which: 0 E, 1 H (global); 2 E, 3 H, 4 J (window); 5 J (global)
+    void setField(int which, const std::vector<double> &f)
+    {
+        check(csrst_upload_field(ctx_, which, f.data(), (int)f.size()), ctx_);
+    }
     std::vector<double> field(int which) const
     {
         int n = 0;
@@ -232,10 +252,18 @@ private:
 
 class PartBunch {
 public:
-    // PartBunch(...) + makeReady(dt/2) + the window set-up of main.cpp:199-347
-    PartBunch(Engine &e, const Bunch &b) : e_(e), n_(b.size())
+    // PartBunch(...) + makeReady(dt/2) + the window set-up of main.cpp:199-347;
+    // with b.oldR given, the restart instead (no makeReady, the window filled
+    // from the global fields as uploaded, the step counter at step)
+    PartBunch(Engine &e, const Bunch &b, int step = 0) : e_(e), n_(b.size())
     {
-        check(csrst_init_bunch(e.ctx(), b.R.data(), b.P.data(), b.Q.data(), (int)n_), e.ctx());
+        if (b.oldR.empty())
+            check(csrst_init_bunch(e.ctx(), b.R.data(), b.P.data(), b.Q.data(), (int)n_), e.ctx());
+        else if (b.oldR.size() != 2 * n_)
+            throw Error(CSRST_EINVAL, "oldR size");
+        else
+            check(csrst_restore_bunch(e.ctx(), b.R.data(), b.oldR.data(), b.P.data(), b.Q.data(), (int)n_, step),
+                  e.ctx());
     }
     size_t getTotalNum() const { return n_; }
     void get_rmean(double out[2]) const { diag(); out[0] = d_[0]; out[1] = d_[1]; }
@@ -246,6 +274,12 @@ public:
     {
         R.resize(2 * n_); P.resize(2 * n_); oldR.resize(2 * n_);
         check(csrst_download_particles(e_.ctx(), R.data(), P.data(), oldR.data(), (int)n_), e_.ctx());
+    }
+    std::vector<double> getCharges() const
+    {
+        std::vector<double> Q(n_);
+        check(csrst_download_charges(e_.ctx(), Q.data(), (int)n_), e_.ctx());
+        return Q;
     }
 
 private:
@@ -397,20 +431,16 @@ private:
 // Flags as TCLAP accepts them: "-x 64", "--numgridpx 64", "--numgridpx=64";
 // switches -I/--no_initials and -d/--debug.  An unknown flag or a bad value
 // throws (the reference prints the TCLAP error and returns EXIT_FAILURE).
-inline Commands Commands::parse(int argc, char **argv)
+inline std::vector<Commands::Opt> Commands::options()
 {
-    Commands c;
-    struct Opt {
-        const char *shortName, *longName;
-        char kind;   // i int, d double, s string, b switch
-        void *dst;
-    };
-    const Opt opts[] = {
+    Commands &c = *this;
+    return {
         {"r", "restart", 's', &c.restart},         {"n", "startstep", 'i', &c.startstep},
         {"N", "numsteps", 'i', &c.numsteps},       {"x", "numgridpx", 'i', &c.numgridpx},
         {"y", "numgridpy", 'i', &c.numgridpy},     {"m", "plotmodulus", 'i', &c.plotmodulus},
         {"", "statmodulus", 'i', &c.statmodulus},  {"", "linemodulus", 'i', &c.linemodulus},
-        {"", "reballance", 'i', &c.reballance},    {"P", "phi", 'd', &c.phi},
+        {"", "reballance", 'i', &c.reballance},    {"M", "savemodulus", 'i', &c.savemodulus},
+        {"P", "phi", 'd', &c.phi},
         {"w", "width", 'd', &c.width},             {"l", "length", 'd', &c.length},
         {"", "lengthbefore", 'd', &c.lengthbefore}, {"E", "Ekin", 'd', &c.Ekin},
         {"B", "Bz", 'd', &c.Bz},                   {"Q", "Qtotal", 'd', &c.Qtotal},
@@ -422,6 +452,12 @@ inline Commands Commands::parse(int argc, char **argv)
         {"", "wake", 's', &c.wake},                {"", "poissontol", 'd', &c.poissontol},
         {"", "poissonmaxit", 'i', &c.poissonmaxit},
     };
+}
+
+inline Commands Commands::parse(int argc, char **argv)
+{
+    Commands c;
+    const std::vector<Opt> opts = c.options();
     for (int k = 1; k < argc; ++k) {
         std::string a = argv[k], val;
         bool hasVal = false;
@@ -436,6 +472,7 @@ inline Commands Commands::parse(int argc, char **argv)
             for (const Opt &x : opts) if (a.substr(1) == x.shortName) o = &x;
         }
         if (!o) throw Error(CSRST_EINVAL, "unknown argument: " + a);
+        c.given.push_back(o->longName);
         if (o->kind == 'b') { *(bool *)o->dst = true; continue; }
         if (!hasVal) {
             if (k + 1 >= argc) throw Error(CSRST_EINVAL, "missing value for " + a);
@@ -455,7 +492,6 @@ inline Commands Commands::parse(int argc, char **argv)
         }
     }
     if (c.order < 0 || c.order > 2) throw Error(CSRST_EINVAL, "--order must be 0, 1 or 2");
-    if (!c.restart.empty()) throw Error(CSRST_EINVAL, "--restart (H5Part restart files) is not supported");
     return c;
 }
 

@@ -6,8 +6,10 @@
 // and the window as Data/mm_NNNN.pvtr at the start and every --plotmodulus
 // steps, after copyFields; Data/c_collection.pvd and mm_collection.pvd at the
 // end.  Every --statmodulus steps prints a summary line and appends the beam
-// statistics to Data/maxwadis.stat (DataSinkCommand).  Writes the CSR wake to
-// --wake FILE.
+// statistics to Data/maxwadis.stat (DataSinkCommand).  -M/--savemodulus saves
+// restart records (SaveH5Part) to --h5file, -r/--restart FILE continues from
+// the last one (LoadH5Part); H5Part names go through csrst_h5part.  Writes the
+// CSR wake to --wake FILE.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -16,6 +18,7 @@
 #include <memory>
 
 #include "csrst.hpp"
+#include "restart.hpp"
 #include "vtk.hpp"
 
 namespace {
@@ -73,42 +76,73 @@ void plot(const csrst::Engine &engine, csrst::vtk::Plot &global, csrst::vtk::Plo
 int main(int argc, char **argv)
 {
     try {
-        const csrst::Commands cmd = csrst::Commands::parse(argc, argv);
+        csrst::Commands cmd = csrst::Commands::parse(argc, argv);
+        // -r FILE: the last step of a restart file and its arguments
+        // (main.cpp:144-153, LoadH5Part.cpp:60-163), read before the GPU is used
+        const bool restarting = !cmd.restart.empty();
+        csrst::ckpt::Record rs;
+        if (restarting) {
+            rs = csrst::restart::load(cmd.restart);
+            csrst::restart::apply(cmd, rs);
+        }
         csrst::Engine engine(cmd);
         csrst::Bend geo(engine);
-        const csrst::Bunch b = cmd.bunch.empty()
-                                   ? csrst::Distribution::generate(cmd, engine.hr(), engine.origin())
-                                   : read_bunch(cmd.bunch);
-        csrst::PartBunch bunch(engine, b);
+        csrst::Bunch b;
+        if (restarting)
+            b = csrst::restart::restore_fields(engine, rs);   // restoreFields + restoreBunch (main.cpp:226-228)
+        else
+            b = cmd.bunch.empty() ? csrst::Distribution::generate(cmd, engine.hr(), engine.origin())
+                                  : read_bunch(cmd.bunch);
+        csrst::PartBunch bunch(engine, b, restarting ? cmd.startstep : 0);
         std::printf("csrst: %s, grid %d x %d, dx %.6g dy %.6g dt %.6g, %zu particles, order %d\n",
                     csrst_version(), cmd.numgridpx, cmd.numgridpy, engine.hr()[0], engine.hr()[1], engine.dt(),
                     bunch.getTotalNum(), cmd.order);
-        if (!cmd.no_initials) {
+        if (restarting)
+            std::printf("csrst: restart from %s, step %d, t %.9e\n", cmd.restart.c_str(), cmd.startstep, rs.time);
+        else if (!cmd.no_initials) {
             csrst::PoissonSolver poisson(engine);
             poisson.computeField(cmd.poissontol, cmd.poissonmaxit);
             std::printf("csrst: Poisson start, V-cycles %d %d %d, residuals %.2e %.2e %.2e\n", poisson.cycles[0],
                         poisson.cycles[1], poisson.cycles[2], poisson.resid[0], poisson.resid[1], poisson.resid[2]);
         }
+        double t = restarting ? rs.time : 0.0;   // currentSimulatedTime
         csrst::TETMSolver solver(engine);
         ::mkdir("Data", 0755);
         // DataSinkCommand writes Data/maxwadis.stat (DataSink.cpp:22-30)
         std::unique_ptr<csrst::DataSink> sink;
         csrst::PartBunchState state;
         if (cmd.statmodulus > 0) sink.reset(new csrst::DataSink(engine, "Data/maxwadis.stat"));
-        // the start state is plotted without a copy (main.cpp:358-366)
+        // restart records every --savemodulus steps, as the SaveH5Part calls
+        // of main.cpp:361-366,425-430 (commented out there) would write them:
+        // into --h5file when -M is given, or appended to the restart file on
+        // a restart; an H5Part name is written by csrst_h5part at the end
+        const bool saving = (cmd.isSet("savemodulus") || restarting) && cmd.savemodulus > 0;
+        const std::string saveTo = restarting ? cmd.restart : cmd.h5file;
+        const bool viaTool = csrst::restart::is_h5part(saveTo);
+        const std::string records = viaTool ? saveTo + ".rs.tmp" + std::to_string(::getpid()) : saveTo;
+        std::unique_ptr<csrst::ckpt::Writer> saver;
+        if (saving) saver.reset(new csrst::ckpt::Writer(records, !viaTool));
+        // the start state is plotted (and saved) without a copy (main.cpp:358-366)
         csrst::vtk::Plot pavc("Data", "c"), mmpavc("Data", "mm");
-        plot(engine, pavc, mmpavc, 0.0);
+        if (!restarting) {
+            plot(engine, pavc, mmpavc, 0.0);
+            if (saving) saver->write(csrst::restart::snapshot(engine, bunch, t, cmd.startstep));
+        }
         if (cmd.linemodulus > 0 && cmd.linemodulus <= cmd.numsteps)
             std::printf("csrst: --linemodulus: the field copy is done, the SaveEMFields line files are not written\n");
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = cmd.startstep; it < cmd.startstep + cmd.numsteps; ++it) {
             solver.updateField_firstStep();
             solver.updateField_secondStep();
-            const double t = (it - cmd.startstep + 1) * engine.dt();   // currentSimulatedTime + dt
+            const double tn = t + engine.dt();   // currentSimulatedTime + dt
             const bool plotStep = cmd.plotmodulus > 0 && it % cmd.plotmodulus == cmd.plotmodulus - 1;
             if (plotStep) {   // main.cpp:416-423
                 engine.copyFields(true);
-                plot(engine, pavc, mmpavc, t);
+                plot(engine, pavc, mmpavc, tn);
+            }
+            if (saving && it % cmd.savemodulus == cmd.savemodulus - 1) {   // main.cpp:425-430
+                engine.copyFields(true);
+                saver->write(csrst::restart::snapshot(engine, bunch, tn, it + 1));
             }
             if (cmd.statmodulus > 0 && it % cmd.statmodulus == cmd.statmodulus - 1) {   // main.cpp:435-444
                 double rm[2], pm[2];
@@ -116,15 +150,23 @@ int main(int argc, char **argv)
                 bunch.get_pmean(pm);
                 std::printf("step %d: rmean %.9e %.9e pmean %.9e %.9e energy %.9e\n", it + 1, rm[0], rm[1], pm[0],
                             pm[1], bunch.fieldEnergy());
-                sink->execute(engine, t, state);
+                sink->execute(engine, tn, state);
             }
             // main.cpp:468-476: the line dump copies E and H first
             if (cmd.linemodulus > 0 && it % cmd.linemodulus == cmd.linemodulus - 1 && !plotStep)
                 engine.copyFields(false);
             engine.moveMesh();   // wake sample, J = 0, moveMesh (main.cpp:445-501)
+            t += engine.dt();
         }
         pavc.close();   // main.cpp:521-522
         mmpavc.close();
+        if (saver) {
+            saver.reset();
+            if (viaTool) {
+                csrst::restart::run_tool("write", records, saveTo);
+                std::remove(records.c_str());
+            }
+        }
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const auto inf = engine.info();
         std::printf("csrst: %d steps in %.3f s, window at (%d, %d) of %d x %d\n", cmd.numsteps, sec, (int)inf[3],

@@ -93,6 +93,18 @@ class TwoGridStep:
         self.n = len(R)
         check(_lib.lib().csrst_init_bunch(self.h, dptr(R), dptr(P), dptr(Q), C.c_int(self.n)), self.h)
 
+    def restore(self, E, H, R, oldR, P, Q, step=0):
+        """Restart (LoadH5Part::restoreFields + restoreBunch, main.cpp:147-153,
+        226-228): global E and H ([2, Ny+2, Nx+2] planes), then the bunch with
+        its oldR (no makeReady); the step counter starts at step."""
+        self.set_field("E", E)
+        self.set_field("H", H)
+        R, O, P = (np.ascontiguousarray(a, np.float64).reshape(-1, 2) for a in (R, oldR, P))
+        Q = np.ascontiguousarray(Q, np.float64)
+        self.n = len(R)
+        check(_lib.lib().csrst_restore_bunch(self.h, dptr(R), dptr(O), dptr(P), dptr(Q), C.c_int(self.n),
+                                             C.c_int(step)), self.h)
+
     def computeField(self, tol=1e-12, maxit=500):
         """PoissonSolver::computeField (PoissonSolver.cpp:85-201) on the GPU:
         the three multigrid solves into the global E and H, resetOutsideDomain
@@ -161,6 +173,11 @@ class TwoGridStep:
         O = np.zeros((self.n, 2))
         check(_lib.lib().csrst_download_particles(self.h, dptr(R), dptr(P), dptr(O), C.c_int(self.n)), self.h)
         return R, P, O
+
+    def charges(self):
+        Q = np.zeros(self.n)
+        check(_lib.lib().csrst_download_charges(self.h, dptr(Q), C.c_int(self.n)), self.h)
+        return Q
 
     def wake(self):
         L = _lib.lib()

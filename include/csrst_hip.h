@@ -76,6 +76,13 @@ int csrst_boundary_cells(csrst_ctx *c, uint8_t *mask, int *ij, double *data, int
  * makeReady(dt/2) (PartBunch.cpp:164-177), global->window copy and
  * boundary exchange (main.cpp:335-347).  R, P: 2 doubles per particle. */
 int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, int n);
+/* Restart (main.cpp:147-153,226-228; LoadH5Part.cpp:174-321): as
+ * csrst_init_bunch with oldR given instead of makeReady, the global E and H
+ * as last uploaded with csrst_upload_field (restoreFields) copied into the
+ * window, global J zeroed (main.cpp:388), and the step counter, which phases
+ * the wake samples (iteration % statModulus), set to step. */
+int csrst_restore_bunch(csrst_ctx *c, const double *R, const double *oldR, const double *P, const double *Q, int n,
+                        int step);
 
 /* Whole steps (main.cpp:390-506). */
 int csrst_step(csrst_ctx *c, int nsteps);
@@ -102,6 +109,8 @@ int csrst_substep(csrst_ctx *c, int op);
 int csrst_download_field(csrst_ctx *c, int which, double *out, int cap, int *count);
 int csrst_upload_field(csrst_ctx *c, int which, const double *in, int count);
 int csrst_download_particles(csrst_ctx *c, double *R, double *P, double *oldR, int n);
+/* the charges (PartBunch::Q) in the order of csrst_download_particles */
+int csrst_download_charges(csrst_ctx *c, double *Q, int n);
 /* info: dt, window origin x, y, window position x, y, W, H, steps, wake samples */
 int csrst_info(csrst_ctx *c, double *info);
 /* CSR wake accumulator of IntegrateElongCommand (IntegrateElong.cpp:38-89). */

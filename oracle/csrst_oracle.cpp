@@ -1182,13 +1182,18 @@ struct Sim {
         E.alloc(p.Nx, p.Ny); H.alloc(p.Nx, p.Ny); J.alloc(p.Nx, p.Ny);
     }
 
-    int init(const double *R0, const double *P0, const double *Q0, int n)
+    /* O0 null: the -I start (makeReady); else the restart of main.cpp:147-153,
+       228 with LoadH5Part::restoreBunch's oldR and no makeReady. */
+    int init(const double *R0, const double *P0, const double *Q0, int n, const double *O0 = nullptr, int step0 = 0)
     {
         N = n;
         R.assign(R0, R0 + 2 * n);
         P.assign(P0, P0 + 2 * n);
         Q.assign(Q0, Q0 + n);
-        oldR.assign(2 * n, 0.0);
+        if (O0)
+            oldR.assign(O0, O0 + 2 * n);
+        else
+            oldR.assign(2 * n, 0.0);
         /* Distribution::getDomain, Distribution.cpp:181-207, then +-5 cells */
         double mn[2] = {R[0], R[1]}, mx[2] = {R[0], R[1]};
         for (int k = 0; k < N; ++k)
@@ -1231,7 +1236,7 @@ struct Sim {
             R.swap(R2); P.swap(P2); oldR.swap(O2); Q.swap(Q2);
         }
         /* makeReady(dt/2), PartBunch.cpp:164-177 */
-        for (int k = 0; k < N; ++k) {
+        for (int k = 0; k < N && !O0; ++k) {
             const V2 Pk = v2(P[2 * k], P[2 * k + 1]);
             const double g = (double)q_rsqrt((float)(1 + dot(Pk, Pk)));
             const double h = dt / 2 * C_LIGHT;
@@ -1250,7 +1255,7 @@ struct Sim {
         wsum.assign(nsamp, 0.0L);
         wcor.assign(nsamp, 0.0L);
         J.c[0].assign(J.c[0].size(), 0.0); J.c[1].assign(J.c[1].size(), 0.0);
-        steps = 0;
+        steps = step0;
         return 0;
     }
 
@@ -2152,6 +2157,15 @@ int orc_sim_init(void *s, const double *R, const double *P, const double *Q, int
         return as<Sim>(s)->init(R, P, Q, n);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "oracle init: %s\n", e.what());
+        return -1;
+    }
+}
+int orc_sim_restore(void *s, const double *R, const double *oldR, const double *P, const double *Q, int n, int step)
+{
+    try {
+        return as<Sim>(s)->init(R, P, Q, n, oldR, step);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle restore: %s\n", e.what());
         return -1;
     }
 }

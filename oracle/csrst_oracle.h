@@ -83,6 +83,10 @@ void  orc_sim_destroy(void *s);
 /* R, P interleaved (x,y) per particle; Q per particle.  Performs the -I start:
  * window sizing, PartBunch sort, makeReady(dt/2), copy global->window, exchange. */
 int   orc_sim_init(void *s, const double *R, const double *P, const double *Q, int n);
+/* The restart: oldR as given (no makeReady), global E/H as set with
+   orc_sim_set_field copied into the window, step counter = step. */
+int   orc_sim_restore(void *s, const double *R, const double *oldR, const double *P, const double *Q, int n,
+                      int step);
 int   orc_sim_step(void *s, int nsteps);
 /* info: dt, window origin x/y, ddim first x/y, W, H, steps done, wake samples */
 void  orc_sim_info(void *s, double *info);

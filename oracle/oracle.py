@@ -266,6 +266,19 @@ class Sim:
         if rc != 0:
             raise RuntimeError("oracle init failed (%d)" % rc)
 
+    def restore(self, E, H, R, oldR, P, Q, step=0):
+        """The restart of main.cpp:147-153,226-228: global E and H, then the
+        bunch with its oldR (LoadH5Part::restoreBunch, no makeReady)."""
+        self.set_field("E", E)
+        self.set_field("H", H)
+        R, O, P = (np.ascontiguousarray(a, np.float64).reshape(-1, 2) for a in (R, oldR, P))
+        Q = np.ascontiguousarray(Q, np.float64)
+        self.n = len(R)
+        rc = lib().orc_sim_restore(C.c_void_p(self.h), _ptr(R), _ptr(O), _ptr(P), _ptr(Q), C.c_int(self.n),
+                                   C.c_int(step))
+        if rc != 0:
+            raise RuntimeError("oracle restore failed (%d)" % rc)
+
     def poisson(self, gamma=0.0):
         """PoissonSolver::computeField + resetOutsideDomain + window copy (after init)."""
         rc = lib().orc_sim_poisson(C.c_void_p(self.h), C.c_double(gamma))

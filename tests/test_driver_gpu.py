@@ -113,3 +113,62 @@ def test_driver_vtk_frames(gpu, tmp_path):
     for name, b in want.items():
         assert (data / name).read_bytes() == b, name
     assert len(times) == 3
+
+
+def read_h5part_last(path, out):
+    """The last step of an H5Part file as the driver's record (csrst_h5part read)."""
+    from tests.test_restart_files import unpack
+    tool = os.path.join(ROOT, "csrst_amd", "csrst_h5part")
+    subprocess.run([tool, "read", str(path), str(out)], check=True)
+    (rec,) = unpack(open(out, "rb").read())
+    return rec
+
+
+def test_driver_save_and_restart(gpu, tmp_path):
+    """-M 3 --h5file run.h5part saves the start and steps 3 and 6 (SaveH5Part
+    after copyFields); -r run.h5part then continues (LoadH5Part: the file's
+    arguments, startstep ((0 + 6) / 3) * 3 = 6, numsteps 6) and appends steps
+    9 and 12.  The last record equals the library restored from step 6 and
+    stepped the same way."""
+    if not os.path.exists(EXE) or not os.path.exists(os.path.join(ROOT, "csrst_amd", "csrst_h5part")):
+        pytest.skip("driver or csrst_h5part not built")
+    if not os.path.exists("/opt/conda/lib/libhdf5.so"):
+        pytest.skip("the HDF5 library csrst_h5part was built against is not installed")
+    R, P, Q = gaussian_bunch(2000, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=17, energy_spread=1e-3)
+    run_driver(tmp_path, R, P, Q, ["--numsteps", "6", "--savemodulus", "3", "--h5file", "run.h5part"])
+    rec6 = read_h5part_last(tmp_path / "run.h5part", tmp_path / "a.rs")
+    attrs = dict(rec6["attrs"])
+    assert rec6["step"] == 2 and (attrs["numsteps"], attrs["savemodulus"], attrs["numgridpx"]) == (6, 3, 1100)
+    print("restart:", EXE, "-r run.h5part", flush=True)
+    r = subprocess.run([EXE, "-r", "run.h5part"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "restart from run.h5part, step 6" in r.stdout
+    rec12 = read_h5part_last(tmp_path / "run.h5part", tmp_path / "b.rs")
+    assert rec12["step"] == 4
+    assert rec12["time"] == pytest.approx(12 * rec6["time"] / 6, rel=1e-12)
+    # the library, restored from the step-6 record
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lengthbefore"], length=GEOM["length"],
+                   width=GEOM["width"], phi=GEOM["phi"], Ekin=GEOM["Ekin"], Bz=GEOM["Bz"], order=1)
+    g = TwoGridStep(cmd)
+    Nx, Ny = 1100, 120
+
+    def guarded(v):
+        a = np.zeros((2, Ny + 2, Nx + 2))
+        a[:, 1:-1, 1:-1] = np.asarray(v).reshape(2, Ny, Nx)
+        return a
+
+    g.restore(guarded(rec6["E"]), guarded(rec6["H"]), np.stack([rec6["x"], rec6["y"]], 1),
+              np.stack([rec6["oldx"], rec6["oldy"]], 1), np.stack([rec6["px"], rec6["py"]], 1), rec6["q"], step=6)
+    for it in range(6, 12):
+        g.updateField_firstStep()
+        g.updateField_secondStep()
+        if it % 3 == 2:
+            g.copyFields(True)
+        if it < 11:
+            g.moveMesh()
+    for k, f in (("E", "E"), ("H", "H"), ("J", "J")):
+        assert np.array_equal(rec12[k], g.field(f)[:, 1:-1, 1:-1].ravel()), k
+    Rg, Pg, Og = g.particles()
+    for k, v in (("x", Rg[:, 0]), ("y", Rg[:, 1]), ("oldx", Og[:, 0]), ("oldy", Og[:, 1]), ("px", Pg[:, 0]),
+                 ("py", Pg[:, 1]), ("q", g.charges())):
+        assert np.array_equal(rec12[k], v), k

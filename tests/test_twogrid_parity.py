@@ -110,3 +110,29 @@ def test_copy_fields_bit_identical(gpu):
             g.step(1)
             o.step(1)
             compare(g, o, "step %d after copy (J %s)" % (k, cur))
+
+
+def test_restore_bit_identical(gpu):
+    """The restart (LoadH5Part::restoreFields + restoreBunch, main.cpp:147-153,
+    226-228) through the C ABI against the oracle: global E and H, the bunch
+    with its oldR (no makeReady) and the step counter, which phases the wake
+    samples; then steps on both."""
+    stat = 3
+    g, o = make(2, n=1500, seed=21, stat=stat)
+    for _ in range(4):
+        g.step(1)
+        o.step(1)
+    g.copyFields(True)
+    E, H = g.field("E"), g.field("H")
+    R, P, Rold = g.particles()
+    Q = g.charges()
+    g2, o2 = make(2, n=10, seed=1, stat=stat)       # fresh steppers, then restored
+    g2.restore(E, H, R, Rold, P, Q, step=4)
+    o2.restore(E, H, R, Rold, P, Q, step=4)
+    assert g2.info()["steps"] == 4 and o2.info()["steps"] == 4
+    compare(g2, o2, "restore")
+    for k in range(5):
+        g2.step(1)
+        o2.step(1)
+        compare(g2, o2, "step %d after restore" % k)
+    assert same(g2.wake(), o2.wake()) and np.abs(g2.wake()).max() > 0
