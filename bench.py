@@ -12,6 +12,7 @@ step (two pushes with deposits, the field sweeps, the window move).
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -47,12 +48,11 @@ def run_mw(cfg, fp32, steps, warmup, R, P, Q, stat):
     return (t1 - t0), timing, wake, info
 
 
-def cpu_baseline(seconds=12.0):
-    """The FP64 CPU restatement of the reference step (oracle, one core) on
+def _oracle_sample(seconds):
+    """The FP64 CPU restatement of the reference step (oracle) on one core, on
     the reference's CPU-runnable workload: global grid of the 10 deg bend at
-    ~50 um (4694 x 595), 1e5 macro-particles, IntCIC; timed for a bounded
-    number of steps."""
-    import numpy as np
+    ~50 um (4694 x 595), 1e5 macro-particles, IntCIC; as many steps as fit in
+    the given seconds.  Returns (particles, cells, steps, seconds)."""
     from oracle import oracle as O
     from csrst_amd.bunch import gaussian_bunch
     Nx, Ny = 4694, 595
@@ -67,11 +67,53 @@ def cpu_baseline(seconds=12.0):
     while time.perf_counter() - t0 < seconds:
         s.step(2)
         steps += 2
-    t = time.perf_counter() - t0
+    return n, Nx * Ny, steps, time.perf_counter() - t0
+
+
+def _oracle_worker(seconds):
+    """Child process of cpu_baseline's all-cores leg: one sample, as JSON."""
+    print(json.dumps(_oracle_sample(seconds)), flush=True)
+
+
+def host_cpu():
+    model = "?"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(seconds=12.0, max_procs=16):
+    """The oracle timed on the host: one core (the reference's single-rank
+    semantics), then one independent sample per core in as many child
+    processes (at most max_procs, the host share of one GPU), summed."""
+    n, cells, steps, t = _oracle_sample(seconds)
+    model, ncpu = host_cpu()
+    k = max(1, min(ncpu, max_procs))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = [subprocess.Popen([sys.executable, "-c", "import bench; bench._oracle_worker(%r)" % seconds],
+                              cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True) for _ in range(k)]
+    agg_p = agg_c = 0.0
+    for p in procs:
+        out, _ = p.communicate(timeout=10 * seconds + 120)
+        if p.returncode != 0:
+            raise RuntimeError("cpu baseline worker failed")
+        pn, pc, ps, pt = json.loads(out.strip().splitlines()[-1])
+        agg_p += pn * ps / pt
+        agg_c += pc * ps / pt
     return dict(value=n * steps / t, unit="particle-steps/s", cores=1, kind="port",
-                cell_updates_per_s=Nx * Ny * steps / t,
+                cell_updates_per_s=cells * steps / t,
                 sample="oracle two-grid step, 4694x595 global grid + bunch window, 1e5 particles IntCIC, "
-                       "%d steps in %.1f s on one host core" % (steps, t))
+                       "%d steps in %.1f s on one host core" % (steps, t),
+                host=dict(cpu=model, cores_available=ncpu),
+                all_cores=dict(value=agg_p, cell_updates_per_s=agg_c, cores=k, processes=k,
+                               sample="the same sample in %d independent processes at once, one per core, "
+                                      "throughputs summed" % k))
 
 
 def copy_probe(mib):
