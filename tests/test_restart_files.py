@@ -184,3 +184,39 @@ def test_rejects_bad_input(tmp_path):
     r = subprocess.run([TOOL, "read", str(tmp_path / "missing.h5part"), str(tmp_path / "y.rs")],
                        capture_output=True, text=True)
     assert r.returncode != 0
+
+
+@pytest.fixture(scope="module")
+def args_tool(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("rsargs") / "restart_args_main")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "restart_args_main.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def restart_args(exe, rs, *args):
+    r = subprocess.run([exe, str(rs)] + list(args), capture_output=True, text=True, check=True)
+    return dict(line.split(" ", 1) for line in r.stdout.splitlines())
+
+
+def test_restart_argument_rules(args_tool, tmp_path):
+    """LoadH5Part::readH5FileAttributes (LoadH5Part.cpp:60-163): arguments on
+    the command line win, except numgridpx, numgridpy, Np and phi, which always
+    come from the file; startstep = ((startstep + numsteps) / savemodulus) *
+    savemodulus of the file's values; switches are int64 0/1."""
+    attrs = [("restart", ""), ("startstep", 4), ("numsteps", 11), ("savemodulus", 5), ("numgridpx", 700),
+             ("numgridpy", 90), ("Np", 1234), ("phi", 12.5), ("width", 0.02), ("Ekin", 80.0),
+             ("no_initials", 1), ("h5file", "old.h5part"), ("order", 2)]
+    rec = make_record(np.random.default_rng(1), 3, 2, 2, 0, 1e-10, attrs)
+    (tmp_path / "r.rs").write_bytes(pack([rec]))
+    a = restart_args(args_tool, tmp_path / "r.rs", "-r", "r.rs", "--numgridpx", "64", "-w", "0.05", "-P", "3",
+                     "--numsteps", "7")
+    assert a["restart"] == "r.rs"                      # set: kept
+    assert a["numgridpx"] == "700" and a["phi"] == "12.5" and a["numgridpy"] == "90" and a["Np"] == "1234"
+    assert a["width"] == "0.050000000000000003"         # set: kept
+    assert a["numsteps"] == "7"                         # set: kept
+    assert a["startstep"] == "15"                       # ((4 + 11) / 5) * 5
+    assert a["Ekin"] == "80" and a["no_initials"] == "1" and a["h5file"] == "old.h5part" and a["order"] == "2"
+    assert a["Bz"] == "0.10000000000000001"             # absent from the file: default
