@@ -14,7 +14,7 @@
 //            uint8 kind ('s' string, 'd' double, 'i' integer), value
 //            ('s': int32 length + bytes; 'd': double; 'i': int64)
 // The attributes are the command-line arguments, as SaveH5Part writes them
-// as file attributes (SaveH5Part.cpp:276-353).
+// as file attributes (SaveH5Part.cpp:272-352).
 #pragma once
 
 #include <cstdint>

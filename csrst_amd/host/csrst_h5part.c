@@ -18,7 +18,7 @@
  *   csrst_h5part write RECORDS OUT.h5part   append every record as a step
  *                                            (a new file gets all attributes;
  *                                            an existing one the reduced set
- *                                            of SaveH5Part.cpp:356-428)
+ *                                            of SaveH5Part.cpp:355-428)
  *   csrst_h5part read IN.h5part RECORDS     the last step and the file
  *                                            attributes as one record
  *
@@ -248,7 +248,7 @@ static void write_step(hid_t file, long k, const record_t *r)
     H5CHK(g);
     const double tns = r->time * 1e9;
     attr_write(g, "TIME", H5T_NATIVE_DOUBLE, &tns, 1);
-    /* at least one particle (SaveH5Part.cpp:94-96, 116-117): 0 and id -1 */
+    /* at least one particle (SaveH5Part.cpp:93, 117, 173): 0 and id -1 */
     const size_t n = (size_t)r->n, nloc = n ? n : 1;
     double *buf = xmalloc(nloc * 8);
     const hsize_t dn = nloc;
@@ -309,7 +309,7 @@ static int do_write(const char *in, const char *out)
         long last;
         next = count_steps(file, &last);
         if (last + 1 > next) next = last + 1;
-        /* SaveH5Part.cpp:356-428: the reduced set, int-valued ones only */
+        /* SaveH5Part.cpp:355-428: the reduced set, int-valued ones only */
         const char *reduced[5] = {"startstep", "numsteps", "savemodulus", "order", "debug"};
         for (int k = 0; k < 5; ++k) {
             const attr_t *a = find_attr(&r, reduced[k]);
@@ -318,7 +318,7 @@ static int do_write(const char *in, const char *out)
     } else {
         file = H5Fcreate(out, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT);
         H5CHK(file);
-        /* SaveH5Part.cpp:276-288 units, then every argument */
+        /* SaveH5Part.cpp:274-286 units, then every argument */
         const char *units[11][2] = {{"tUnit", "s"},      {"xUnit", "m"},           {"yUnit", "m"},
                                     {"zUnit", "m"},      {"pxUnit", "#beta#gamma"}, {"pyUnit", "#beta#gamma"},
                                     {"pzUnit", "#beta#gamma"}, {"qUnit", "Cb"},    {"idUnit", "1"},
@@ -417,7 +417,7 @@ static int do_read(const char *in, const char *out)
     if (ta < 0) die("missing TIME", name);
     H5CHK(H5Aread(ta, H5T_NATIVE_DOUBLE, &r.time));
     H5Aclose(ta);
-    r.time *= 1e-9;   /* LoadH5Part.cpp:182 */
+    r.time *= 1e-9;   /* LoadH5Part.cpp:181 */
     size_t n;
     r.x = read_dset(g, "x", H5T_NATIVE_DOUBLE, 0, &n);
     r.n = (int64_t)n;

@@ -56,7 +56,7 @@ inline void run_tool(const char *mode, const std::string &a, const std::string &
         throw Error(CSRST_EINVAL, std::string("csrst_h5part ") + mode + " failed for " + a);
 }
 
-// LoadH5Part::openFile + the last step (LoadH5Part.cpp:49-58)
+// LoadH5Part::openFile + the last step (LoadH5Part.cpp:48-58)
 inline ckpt::Record load(const std::string &path)
 {
     if (!is_h5part(path)) return ckpt::read_all(path).back();
@@ -67,7 +67,7 @@ inline ckpt::Record load(const std::string &path)
     return r;
 }
 
-// LoadH5Part::readH5FileAttributes (LoadH5Part.cpp:60-163): an argument set
+// LoadH5Part::readH5FileAttributes (LoadH5Part.cpp:60-159): an argument set
 // on the command line keeps its value unless it is numgridpx, numgridpy, Np
 // or phi; the others take the file's.  startstep becomes
 // ((startstep + numsteps) / savemodulus) * savemodulus of the file's values.
@@ -94,7 +94,7 @@ inline void apply(Commands &c, const ckpt::Record &r)
 }
 
 // the arguments as SaveH5Part::writeH5FileAttributes stores them
-// (SaveH5Part.cpp:289-353): strings, float64, int64 (switches 0/1)
+// (SaveH5Part.cpp:288-352): strings, float64, int64 (switches 0/1)
 inline std::vector<ckpt::Attr> attributes(Commands c)
 {
     std::vector<ckpt::Attr> out;
@@ -150,9 +150,9 @@ inline ckpt::Record snapshot(const Engine &e, const PartBunch &b, double t, int 
     return r;
 }
 
-// LoadH5Part::restoreFields (LoadH5Part.cpp:165-246) into the engine's global
+// LoadH5Part::restoreFields (LoadH5Part.cpp:161-240) into the engine's global
 // E and H (J is zeroed before the first step, main.cpp:388) and the bunch of
-// restoreBunch (LoadH5Part.cpp:248-321)
+// restoreBunch (LoadH5Part.cpp:242-321)
 inline Bunch restore_fields(Engine &e, const ckpt::Record &r)
 {
     const Commands &c = e.commands();

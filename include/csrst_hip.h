@@ -76,7 +76,7 @@ int csrst_boundary_cells(csrst_ctx *c, uint8_t *mask, int *ij, double *data, int
  * makeReady(dt/2) (PartBunch.cpp:164-177), global->window copy and
  * boundary exchange (main.cpp:335-347).  R, P: 2 doubles per particle. */
 int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const double *Q, int n);
-/* Restart (main.cpp:147-153,226-228; LoadH5Part.cpp:174-321): as
+/* Restart (main.cpp:147-153,226-228; LoadH5Part.cpp:161-321): as
  * csrst_init_bunch with oldR given instead of makeReady, the global E and H
  * as last uploaded with csrst_upload_field (restoreFields) copied into the
  * window, global J zeroed (main.cpp:388), and the step counter, which phases

@@ -10,7 +10,7 @@ through csrst_h5part to an H5Part file and back.
 - reading takes the last step (LoadH5Part.cpp:49-58) and gives back that
   record bit for bit, with the arguments;
 - writing into an existing file appends steps after the last one and updates
-  the reduced argument set (SaveH5Part.cpp:356-428).
+  the reduced argument set (SaveH5Part.cpp:355-428).
 
 H5hut itself is not available here: the layout follows its documented file
 format; parity with an H5hut-written file is unpinned.
@@ -202,7 +202,7 @@ def restart_args(exe, rs, *args):
 
 
 def test_restart_argument_rules(args_tool, tmp_path):
-    """LoadH5Part::readH5FileAttributes (LoadH5Part.cpp:60-163): arguments on
+    """LoadH5Part::readH5FileAttributes (LoadH5Part.cpp:60-159): arguments on
     the command line win, except numgridpx, numgridpy, Np and phi, which always
     come from the file; startstep = ((startstep + numsteps) / savemodulus) *
     savemodulus of the file's values; switches are int64 0/1."""
