@@ -78,7 +78,7 @@ int main(int argc, char **argv)
     try {
         csrst::Commands cmd = csrst::Commands::parse(argc, argv);
         // -r FILE: the last step of a restart file and its arguments
-        // (main.cpp:144-153, LoadH5Part.cpp:60-163), read before the GPU is used
+        // (main.cpp:144-153, LoadH5Part.cpp:60-159), read before the GPU is used
         const bool restarting = !cmd.restart.empty();
         csrst::ckpt::Record rs;
         if (restarting) {
