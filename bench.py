@@ -70,6 +70,32 @@ def _oracle_sample(seconds):
     return n, Nx * Ny, steps, time.perf_counter() - t0
 
 
+def parity_mode_gpu(steps=200, warmup=10):
+    """The reference's own two-grid FP64 step (csrst_ctx, bit-identical to the
+    oracle) on the GPU, on the workload of the CPU baseline: the 4694 x 595
+    global grid of the 10 deg bend plus the bunch window, 1e5 particles,
+    IntCIC.  The like-for-like GPU figure beside the CPU one."""
+    from csrst_amd import Commands, TwoGridStep
+    from csrst_amd.bunch import gaussian_bunch
+    Nx, Ny = 4694, 595
+    cmd = Commands(numgridpx=Nx, numgridpy=Ny, lengthbefore=0.05, length=0.01, width=0.0128, phi=10.0,
+                   Ekin=50.59, Bz=0.1704, order=0, statmodulus=100000)
+    n = 100000
+    R, P, Q = gaussian_bunch(n, (0.03, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=1)
+    g = TwoGridStep(cmd)
+    g.init_bunch(R, P, Q)
+    g.step(warmup)
+    t0 = time.perf_counter()
+    g.step(steps)          # synchronises before returning
+    t = time.perf_counter() - t0
+    inf = g.info()
+    g.close()
+    return dict(particle_steps_per_s=n * steps / t, cell_updates_per_s=(Nx * Ny + inf["W"] * inf["H"]) * steps / t,
+                ms_per_step=1e3 * t / steps, steps=steps,
+                workload="two-grid FP64 (parity mode), 4694x595 global grid + %dx%d window, 1e5 particles IntCIC"
+                         % (inf["W"], inf["H"]))
+
+
 def _oracle_worker(seconds):
     """Child process of cpu_baseline's all-cores leg: one sample, as JSON."""
     print(json.dumps(_oracle_sample(seconds)), flush=True)
@@ -212,6 +238,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()
+        cpu["gpu_same_workload"] = parity_mode_gpu()
     rec = {
         "metric": "particle-steps/s + cell-updates/s on 1 MI355X, FP32 vs FP64 at wake err <=1e-4",
         "value": ps32,

@@ -85,6 +85,8 @@ struct BBox {
     unsigned long long v[4];
 };
 
+// Box of R (and oldR) of the workgroup's particles into *bb.  Called by every
+// thread of the workgroup (blockDim.x <= kBlock, a multiple of 64).
 __device__ inline void bbox_push(BBox *bb, double x, double y, double ox, double oy, bool withOld, bool live)
 {
     double mnx = x, mny = y, mxx = x, mxy = y;
@@ -97,7 +99,18 @@ __device__ inline void bbox_push(BBox *bb, double x, double y, double ox, double
     }
     unsigned long long a = hu::ord(mnx), b = hu::ord(mny), c = hu::ord(mxx), d = hu::ord(mxy);
     a = hu::wave_min(a); b = hu::wave_min(b); c = hu::wave_max(c); d = hu::wave_max(d);
+    // one set of atomics per workgroup: atomics on one address serialise
+    __shared__ unsigned long long part[4][kBlock / 64];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        part[0][w] = a; part[1][w] = b; part[2][w] = c; part[3][w] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            a = min(a, part[0][k]); b = min(b, part[1][k]);
+            c = max(c, part[2][k]); d = max(d, part[3][k]);
+        }
         atomicMin(&bb->v[0], a); atomicMin(&bb->v[1], b);
         atomicMax(&bb->v[2], c); atomicMax(&bb->v[3], d);
     }
