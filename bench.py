@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
+    ap.add_argument("--parity-mode", action="store_true",
+                    help="only time the two-grid FP64 step on the CPU baseline's workload and print it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB, to calibrate HBM counters")
     a = ap.parse_args()
@@ -206,6 +208,9 @@ def main():
         sys.exit(1)
     if a.copy_probe:
         copy_probe(a.copy_probe)
+    if a.parity_mode:
+        print(json.dumps(parity_mode_gpu(steps=a.steps, warmup=a.warmup)), flush=True)
+        return
     cfg = config(a.config)
     n = a.particles or cfg["n"]
     print("bench: config %s, %d particles, %dx%d cells, %d+%d steps" % (a.config, n, cfg["W"], cfg["H"], a.warmup,

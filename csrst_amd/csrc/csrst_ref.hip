@@ -129,21 +129,26 @@ struct Parts {
 };
 
 // push_particles, first loop: oldR = R; R += (0.5 dt c) P / gamma.  PartBunch.cpp:226-232
+// Grid-stride over the particles (a capped grid: fewer workgroups, fewer
+// same-address box atomics); the box of each thread's particles is kept in
+// registers and pushed once.
 __global__ void k_drift_half(Parts p, double half, BBox *bb)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = k < p.n;
-    double x = 0, y = 0;
-    if (live) {
-        x = p.rx[k]; y = p.ry[k];
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    double mnx = inf, mny = inf, mxx = -inf, mxy = -inf;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < p.n; k += gridDim.x * blockDim.x) {
+        double x = p.rx[k], y = p.ry[k];
         p.ox[k] = x; p.oy[k] = y;
         const vec2<double> P = pm::mk(p.px[k], p.py[k]);
         const double g = pm::inv_gamma(P);
         x += half * P.x * g;
         y += half * P.y * g;
         p.rx[k] = x; p.ry[k] = y;
+        mnx = fmin(mnx, x); mny = fmin(mny, y);
+        mxx = fmax(mxx, x); mxy = fmax(mxy, y);
     }
-    bbox_push(bb, x, y, 0, 0, false, live);
+    const bool any = mnx <= mxx;
+    bbox_push(bb, any ? mnx : 0, any ? mny : 0, any ? mxx : 0, any ? mxy : 0, true, any);
 }
 
 // drift_particles, PartBunch.cpp:179-187
@@ -230,7 +235,11 @@ __global__ void k_push_kick(Parts p, FieldSet fs, geom::Bend bend, PushArgs a, c
 struct DepArgs {
     double mmo[2], hr[2], dt;
     int order, comp, W, H;
+    int slots;   // nodes per segment of the order's current stencils (2, 6, 12)
 };
+
+// nodes of one segment's current stencil: S0 2, S1 6, S2 12
+inline int seg_nodes(int order) { return order == 0 ? 2 : order == 1 ? 6 : kMaxNodes; }
 
 // Current of every (particle, segment, node) into fixed slots; keys are
 // window-interior node indices, nodes outside the window are dropped
@@ -239,9 +248,9 @@ __global__ void k_emit(Parts p, DepArgs a, unsigned *keys, double *vals, int *er
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= p.n) return;
-    unsigned *kk = keys + (size_t)k * kMaxSeg * kMaxNodes;
-    double *vv = vals + (size_t)k * kMaxSeg * kMaxNodes;
-    for (int s = 0; s < kMaxSeg * kMaxNodes; ++s) kk[s] = 0xffffffffu;
+    unsigned *kk = keys + (size_t)k * kMaxSeg * a.slots;
+    double *vv = vals + (size_t)k * kMaxSeg * a.slots;
+    for (int s = 0; s < kMaxSeg * a.slots; ++s) kk[s] = 0xffffffffu;
     const vec2<double> org = pm::mk(a.mmo[0], a.mmo[1]);
     const vec2<double> dx = pm::mk(a.hr[0], a.hr[1]);
     const vec2<double> o = pm::mk(p.ox[k] - org.x, p.oy[k] - org.y);
@@ -263,11 +272,12 @@ __global__ void k_emit(Parts p, DepArgs a, unsigned *keys, double *vals, int *er
         const vec2<double> seg = pm::mk(c.s[s + 1].x - c.s[s].x, c.s[s + 1].y - c.s[s].y);
         pm::SegCurrent<double> sc;
         pm::segment_current(sc, a.order, a.comp, xi, xf, seg, covered, q, odt, delta);
+        if (sc.nx * sc.ny > a.slots) { atomicOr(err, 2); return; }
         int t = 0;
         for (int jj = 0; jj < sc.ny; ++jj)
             for (int ii = 0; ii < sc.nx; ++ii, ++t) {
                 const int I = sc.I0 + ii, J = sc.J0 + jj;
-                const int slot = s * kMaxNodes + t;
+                const int slot = s * a.slots + t;
                 if (I >= 0 && I < a.W && J >= 0 && J < a.H) {
                     kk[slot] = (unsigned)(J * a.W + I);
                     vv[slot] = sc.w[t];
@@ -296,9 +306,9 @@ __global__ void k_reduce_runs(const unsigned *keys, const double *vals, size_t c
     const unsigned key = keys[t];
     if (key == 0xffffffffu) return;
     if (t > 0 && keys[t - 1] == key) return;
-    // the same sequential sum, with the loads of eight entries issued ahead
+    // the same sequential sum, with the loads of 16 entries issued ahead
     // of their additions
-    constexpr int G = 8;
+    constexpr int G = 16;
     double s = 0.0;
     for (size_t u = t;; u += G) {
         unsigned kv[G];
@@ -1020,7 +1030,7 @@ void push(csrst_ctx *c, int comp)
     const double half = 0.5 * c->dt * pm::kC;
     k_bbox_reset<<<1, 1, 0, c->s>>>(c->bb.p);
     k_bbox_reset<<<1, 1, 0, c->s>>>(c->bb.p + 1);
-    k_drift_half<<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, half, c->bb.p);
+    k_drift_half<<<std::min(blocks(c->n), 128), kBlock, 0, c->s>>>(c->parts, half, c->bb.p);
     PushArgs a{};
     a.mmo[0] = c->mmo[0]; a.mmo[1] = c->mmo[1];
     a.hr[0] = c->hr[0]; a.hr[1] = c->hr[1];
@@ -1038,9 +1048,12 @@ void push(csrst_ctx *c, int comp)
     d.mmo[0] = c->mmo[0]; d.mmo[1] = c->mmo[1];
     d.hr[0] = c->hr[0]; d.hr[1] = c->hr[1];
     d.dt = c->dt; d.order = c->p.order; d.comp = comp; d.W = c->W; d.H = c->H;
+    d.slots = seg_nodes(c->p.order);
     k_emit<<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, d, c->keys.p, c->vals.p, c->derr.p);
     HU_LAUNCH_CHECK();
-    const size_t cnt = (size_t)c->n * kMaxSeg * kMaxNodes;
+    // (particle, segment, node) slots in that order: a stable sort keeps each
+    // node's contributions in particle order, whatever the slots per segment
+    const size_t cnt = (size_t)c->n * kMaxSeg * d.slots;
     // node keys are below W H; the empty-slot key 0xffffffff has all of the
     // low endBit bits set, so sorting on those bits alone (a stable sort)
     // gives the same order as on all 32 in fewer passes
