@@ -460,22 +460,38 @@ __global__ void k_pack(LayerSet ls, FieldSet fs, double *stage)
     stage[t] = l.op == 0 ? f.c0[q] + f.c1[q] : (l.op == 1 ? f.c0[q] : f.c1[q]);
 }
 
-__global__ void k_unpack(LayerSet ls, FieldSet fs, const double *stage, int k)
+// All layers in one launch.  Where destinations overlap, the value of the
+// last layer writing that component there is kept (the order of the
+// exchanges' layers), so an element is skipped when a later layer writes the
+// same component of the same field at the same node.
+__global__ void k_unpack(LayerSet ls, FieldSet fs, const double *stage)
 {
-    // layers are unpacked one launch per layer, in order, so overlapping
-    // destinations end with the value of the later layer
-    const Layer &l = ls.L[k];
-    const int w = l.hi0 - l.lo0 + 1;
-    const int cnt = w * (l.hi1 - l.lo1 + 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = l.lo0 + t % w + l.sh0, j = l.lo1 + t / w + l.sh1;
+    if (t >= ls.total) return;
+    int k = 0;
+    while (k + 1 < ls.n && ls.L[k + 1].off <= t) ++k;
+    const Layer &l = ls.L[k];
+    const int w = l.hi0 - l.lo0 + 1, e = t - l.off;
+    const int i = l.lo0 + e % w + l.sh0, j = l.lo1 + e / w + l.sh1;
+    bool w0 = l.op != 2, w1 = l.op != 1;   // components this layer writes
+    for (int k2 = k + 1; k2 < ls.n; ++k2) {
+        const Layer &m = ls.L[k2];
+        if (m.dst != l.dst) continue;
+        if (i < m.lo0 + m.sh0 || i > m.hi0 + m.sh0 || j < m.lo1 + m.sh1 || j > m.hi1 + m.sh1) continue;
+        if (m.op != 2) w0 = false;
+        if (m.op != 1) w1 = false;
+    }
     const FieldDesc &f = fs.f[l.dst];
     const size_t q = fidx(f, i, j);
-    const double v = stage[l.off + t];
-    if (l.op == 0) { f.c0[q] = 0.5 * v; f.c1[q] = 0.5 * v; }
-    else if (l.op == 1) f.c0[q] = v;
-    else f.c1[q] = v;
+    const double v = stage[t];
+    if (l.op == 0) {
+        if (w0) f.c0[q] = 0.5 * v;
+        if (w1) f.c1[q] = 0.5 * v;
+    } else if (l.op == 1) {
+        if (w0) f.c0[q] = v;
+    } else if (w1) {
+        f.c1[q] = v;
+    }
 }
 
 // moveMesh copy: dst(i, j) = snapshot(i + mi, j + mj) over the region.
@@ -944,12 +960,8 @@ void run_layers(csrst_ctx *c, const LayerSet &ls)
     if ((size_t)ls.total > c->stage.n) c->stage.alloc((size_t)ls.total * 2);
     k_pack<<<blocks(ls.total), kBlock, 0, c->s>>>(ls, c->fs, c->stage.p);
     HU_LAUNCH_CHECK();
-    for (int k = 0; k < ls.n; ++k) {
-        const Layer &l = ls.L[k];
-        const int cnt = (l.hi0 - l.lo0 + 1) * (l.hi1 - l.lo1 + 1);
-        k_unpack<<<blocks(cnt), kBlock, 0, c->s>>>(ls, c->fs, c->stage.p, k);
-        HU_LAUNCH_CHECK();
-    }
+    k_unpack<<<blocks(ls.total), kBlock, 0, c->s>>>(ls, c->fs, c->stage.p);
+    HU_LAUNCH_CHECK();
 }
 
 void exchange(csrst_ctx *c, int op)
