@@ -1,6 +1,6 @@
 // Binary VTK field output of the reference's PlotBinaryVTKCommand
-// (PlotBinaryVTK.cpp:29-88, PlotBinaryVTK.hh:57-80) and BinaryVtkFile
-// (BinaryVTKFile.hh:60-381, BinaryVTKFile.cpp:21-217), for one rank.
+// (PlotBinaryVTK.cpp:30-88, PlotBinaryVTK.hh:58-80) and BinaryVtkFile
+// (BinaryVTKFile.hh:25-381, BinaryVTKFile.cpp:25-217), for one rank.
 //
 // Every execute() writes
 //   Data/<base>_NNNN_node00000.vtr  rectilinear-grid piece: XML header, then
@@ -80,8 +80,8 @@ inline std::string extent(int nx, int ny)
     return s.str();
 }
 
-// The piece file (BinaryVtkFile::writeFile, BinaryVTKFile.cpp:21-52).
-// H-Field: Hzx + Hzy over [0, nx-2] x [0, ny-2], as PlotBinaryVTK.cpp:68-76
+// The piece file (BinaryVtkFile::writeFile, BinaryVTKFile.cpp:25-52).
+// H-Field: Hzx + Hzy over [0, nx-2] x [0, ny-2], as PlotBinaryVTK.cpp:72-80
 // fills it; the reference leaves the last column and row of that scalar field
 // as the new field holds them, written as 0 here.
 inline std::string piece(const Frame &f)
@@ -149,7 +149,7 @@ inline std::string piece(const Frame &f)
     return out;
 }
 
-// The parallel header (BinaryVtkFile::writeHeaderFile, BinaryVTKFile.cpp:54-86);
+// The parallel header (BinaryVtkFile::writeHeaderFile, BinaryVTKFile.cpp:54-78);
 // source: the piece's file name without its directory.
 inline std::string header(int nx, int ny, const std::string &source)
 {
@@ -174,7 +174,7 @@ inline std::string header(int nx, int ny, const std::string &source)
     return x.str();
 }
 
-// The time collection (PlotBinaryVTKCommand::closeFile, PlotBinaryVTK.hh:57-80);
+// The time collection (PlotBinaryVTKCommand::closeFile, PlotBinaryVTK.hh:58-80);
 // times printed as an ostream prints a double (six significant digits).
 inline std::string collection(const std::string &base, const std::vector<double> &times)
 {
@@ -204,7 +204,7 @@ inline void write_file(const std::string &path, const std::string &bytes)
 // PlotBinaryVTKCommand for one rank: execute(t, frame) writes frame number
 // k = 0, 1, ... as dir/<base>_kkkk.pvtr and its piece; close() the collection.
 // The constructor counts the DataSet lines of dir/collection.pvd, as the
-// reference's does (PlotBinaryVTK.cpp:91-119), so frame numbers continue
+// reference's does (PlotBinaryVTK.cpp:90-119), so frame numbers continue
 // after those; the reference never writes a file of that name itself.
 class Plot {
 public:

@@ -1,6 +1,6 @@
 """CPU restatement of the reference's binary VTK field output for one rank:
-BinaryVtkFile (BinaryVTKFile.hh:60-381, BinaryVTKFile.cpp:21-217) as
-PlotBinaryVTKCommand drives it (PlotBinaryVTK.cpp:29-88, PlotBinaryVTK.hh:57-80).
+BinaryVtkFile (BinaryVTKFile.hh:25-381, BinaryVTKFile.cpp:25-217) as
+PlotBinaryVTKCommand drives it (PlotBinaryVTK.cpp:30-88, PlotBinaryVTK.hh:58-80).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the driver's
 writer.  The product never imports this module.

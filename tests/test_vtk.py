@@ -136,7 +136,7 @@ def test_cpp_writer_bytes_equal_oracle(writer, tmp_path, nx, ny):
 
 def test_cpp_writer_continues_after_collection_file(writer, tmp_path):
     """The reference's constructor counts the DataSets of Data/collection.pvd
-    (PlotBinaryVTK.cpp:91-119): frames continue after them."""
+    (PlotBinaryVTK.cpp:90-119): frames continue after them."""
     E, J, H = fields(np.random.default_rng(3), 4, 3)
     np.concatenate([E, J, H]).tofile(str(tmp_path / "f.bin"))
     out = tmp_path / "Data"
