@@ -23,7 +23,7 @@ SYMBOLS = [
     "csrst_step", "csrst_first_step", "csrst_second_step", "csrst_move_window", "csrst_copy_fields", "csrst_substep",
     "csrst_download_field", "csrst_upload_field", "csrst_download_particles", "csrst_download_charges",
     "csrst_info",
-    "csrst_wake", "csrst_diag", "csrst_beam_stats", "csrst_field_energy",
+    "csrst_wake", "csrst_wake_reset", "csrst_is_inside_bend", "csrst_diag", "csrst_beam_stats", "csrst_field_energy",
     "csrst_mw_create", "csrst_mw_destroy", "csrst_mw_init_bunch", "csrst_mw_step", "csrst_mw_place",
     "csrst_mw_upload_fields",
     "csrst_mw_download_fields", "csrst_mw_download_particles", "csrst_mw_beam_stats", "csrst_mw_info", "csrst_mw_wake",

@@ -1782,6 +1782,30 @@ int csrst_wake(csrst_ctx *c, double *out, int cap, int *count)
     }
 }
 
+int csrst_wake_reset(csrst_ctx *c)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "not initialised");
+        c->wacc.zero(c->s);
+        HU_CHECK(hipStreamSynchronize(c->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// Bend::isInsideBend (Bend.hh:307-323): rotate R - center by phi; inside when
+// x of R - center is positive and x of the rotated vector negative
+int csrst_is_inside_bend(csrst_ctx *c, const double *R, int *inside)
+{
+    if (!c || !R || !inside) return CSRST_EINVAL;
+    const geom::Bend &b = c->bend;
+    const double dx0 = R[0] - b.center[0], dx1 = R[1] - b.center[1];
+    const double rot0 = cos(b.phi) * dx0 - sin(b.phi) * dx1;
+    *inside = (dx0 < 0.0 || rot0 > 0.0) ? 0 : (dx0 > 0.0 && rot0 < 0.0) ? 1 : 0;
+    return CSRST_OK;
+}
+
 // particle source of the beam statistics: the FP64 SoA bunch
 struct BeamSrc {
     const double *rx, *ry, *px, *py;

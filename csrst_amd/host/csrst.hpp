@@ -189,6 +189,24 @@ public:
         w.resize(n);
         return w;
     }
+    // IntegrateElongCommand::reset (IntegrateElong.hh:59-62)
+    void wakeReset() { check(csrst_wake_reset(ctx_), ctx_); }
+    // IntegrateElongCommand::write (IntegrateElong.cpp:91-125): per sample k
+    // (window column k + 2) its x relative to the bunch centroid and
+    // -statModulus dt times the accumulated field, as "x\tvalue" lines
+    // printed with the stream's default six digits
+    void writeWake(const std::string &path, double rmeanX) const
+    {
+        const std::vector<double> w = wake();
+        const std::array<double, 9> inf = info();
+        const double h = hr()[0], wdt = cmd_.statmodulus * dt();
+        std::ofstream out(path);
+        if (!out) throw Error(CSRST_EINVAL, "cannot write " + path);
+        for (size_t k = 0; k < w.size(); ++k) {
+            const double curPos = inf[1] + (double)(k + 2) * h - rmeanX;
+            out << curPos << "\t" << -wdt * w[k] << "\n";
+        }
+    }
     // dt, window origin (2), window position (2), W, H, steps, wake samples
     std::array<double, 9> info() const
     {
@@ -223,6 +241,13 @@ public:
     double getAngle() const { return e_.phi(); }
     std::array<double, 2> getMeshSpacing() const { return {e_.hr()[0], e_.hr()[1]}; }
     std::array<double, 2> getOrigin() const { return {e_.origin()[0], e_.origin()[1]}; }
+    // Bend.hh:307-323
+    bool isInsideBend(const double R[2]) const
+    {
+        int in = 0;
+        check(csrst_is_inside_bend(e_.ctx(), R, &in), e_.ctx());
+        return in != 0;
+    }
     // Bend.hh:288-291
     size_t getLengthStraightSection() const { return (size_t)(e_.commands().lengthbefore / e_.hr()[0]); }
     std::vector<uint8_t> getInsideMask() const

@@ -6,7 +6,8 @@
 // and the window as Data/mm_NNNN.pvtr at the start and every --plotmodulus
 // steps, after copyFields; Data/c_collection.pvd and mm_collection.pvd at the
 // end.  Every --statmodulus steps prints a summary line and appends the beam
-// statistics to Data/maxwadis.stat (DataSinkCommand).  -M/--savemodulus saves
+// statistics to Data/maxwadis.stat (DataSinkCommand); the CSR wake goes to
+// Data/energyChange{Before,InBend,After}.dat (IntegrateElongCommand::write).  -M/--savemodulus saves
 // restart records (SaveH5Part) to --h5file, -r/--restart FILE continues from
 // the last one (LoadH5Part); H5Part names go through csrst_h5part.  Writes the
 // CSR wake to --wake FILE.
@@ -130,6 +131,7 @@ int main(int argc, char **argv)
         }
         if (cmd.linemodulus > 0 && cmd.linemodulus <= cmd.numsteps)
             std::printf("csrst: --linemodulus: the field copy is done, the SaveEMFields line files are not written\n");
+        bool inStraightSectionBeforeBend = true, inBend = false;   // main.cpp:91-92
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = cmd.startstep; it < cmd.startstep + cmd.numsteps; ++it) {
             solver.updateField_firstStep();
@@ -151,6 +153,21 @@ int main(int argc, char **argv)
                 std::printf("step %d: rmean %.9e %.9e pmean %.9e %.9e energy %.9e\n", it + 1, rm[0], rm[1], pm[0],
                             pm[1], bunch.fieldEnergy());
                 sink->execute(engine, tn, state);
+                // the wake files at the bend's entry and exit (main.cpp:445-463);
+                // the wake is then sampled in moveMesh (main.cpp:465)
+                const double rmax[2] = {state.get_rmax(0), state.get_rmax(1)};
+                const double rmin[2] = {state.get_rmin(0), state.get_rmin(1)};
+                if (inStraightSectionBeforeBend && geo.isInsideBend(rmax)) {
+                    engine.writeWake("Data/energyChangeBefore.dat", state.get_rmean(0));
+                    engine.wakeReset();
+                    inStraightSectionBeforeBend = false;
+                    inBend = true;
+                }
+                if (inBend && !geo.isInsideBend(rmax) && !geo.isInsideBend(rmin)) {
+                    engine.writeWake("Data/energyChangeInBend.dat", state.get_rmean(0));
+                    engine.wakeReset();
+                    inBend = false;
+                }
             }
             // main.cpp:468-476: the line dump copies E and H first
             if (cmd.linemodulus > 0 && it % cmd.linemodulus == cmd.linemodulus - 1 && !plotStep)
@@ -158,6 +175,8 @@ int main(int argc, char **argv)
             engine.moveMesh();   // wake sample, J = 0, moveMesh (main.cpp:445-501)
             t += engine.dt();
         }
+        csrst::calcBeamParameters(engine, state);   // main.cpp:516-519
+        engine.writeWake("Data/energyChangeAfter.dat", state.get_rmean(0));
         pavc.close();   // main.cpp:521-522
         mmpavc.close();
         if (saver) {

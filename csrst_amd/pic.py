@@ -174,6 +174,17 @@ class TwoGridStep:
         check(_lib.lib().csrst_download_particles(self.h, dptr(R), dptr(P), dptr(O), C.c_int(self.n)), self.h)
         return R, P, O
 
+    def wake_reset(self):
+        """IntegrateElongCommand::reset: the wake accumulator to zero."""
+        check(_lib.lib().csrst_wake_reset(self.h), self.h)
+
+    def isInsideBend(self, R):
+        """Bend::isInsideBend (Bend.hh:307-323)."""
+        r = np.ascontiguousarray(R, np.float64)
+        out = C.c_int(0)
+        check(_lib.lib().csrst_is_inside_bend(self.h, dptr(r), C.byref(out)), self.h)
+        return bool(out.value)
+
     def charges(self):
         Q = np.zeros(self.n)
         check(_lib.lib().csrst_download_charges(self.h, dptr(Q), C.c_int(self.n)), self.h)

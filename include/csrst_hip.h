@@ -115,6 +115,11 @@ int csrst_download_charges(csrst_ctx *c, double *Q, int n);
 int csrst_info(csrst_ctx *c, double *info);
 /* CSR wake accumulator of IntegrateElongCommand (IntegrateElong.cpp:38-89). */
 int csrst_wake(csrst_ctx *c, double *out, int cap, int *count);
+/* IntegrateElongCommand::reset (IntegrateElong.hh:59-62): the accumulator to zero,
+ * as main.cpp:445-463 does after writing it at the bend's entry and exit. */
+int csrst_wake_reset(csrst_ctx *c);
+/* Bend::isInsideBend (Bend.hh:307-323) at R[0..1]: *inside = 1 or 0. */
+int csrst_is_inside_bend(csrst_ctx *c, const double *R, int *inside);
 /* rmean(2), pmean(2), qtotal, field energy of the global grid */
 int csrst_diag(csrst_ctx *c, double *out);
 /* Beam parameters of PartBunch::calcBeamParameters / calcMoments

@@ -108,7 +108,7 @@ def test_driver_vtk_frames(gpu, tmp_path):
     want["c_collection.pvd"] = V.collection_bytes("c", times)
     want["mm_collection.pvd"] = V.collection_bytes("mm", times)
     data = tmp_path / "Data"
-    got = sorted(n for n in os.listdir(data) if n != "maxwadis.stat")
+    got = sorted(n for n in os.listdir(data) if n.endswith((".vtr", ".pvtr", ".pvd")))
     assert got == sorted(want)
     for name, b in want.items():
         assert (data / name).read_bytes() == b, name
@@ -172,3 +172,50 @@ def test_driver_save_and_restart(gpu, tmp_path):
     for k, v in (("x", Rg[:, 0]), ("y", Rg[:, 1]), ("oldx", Og[:, 0]), ("oldy", Og[:, 1]), ("px", Pg[:, 0]),
                  ("py", Pg[:, 1]), ("q", g.charges())):
         assert np.array_equal(rec12[k], v), k
+
+
+def test_driver_wake_files(gpu, tmp_path):
+    """IntegrateElongCommand::write (IntegrateElong.cpp:91-125) as main.cpp
+    drives it: Data/energyChangeBefore.dat when the bunch's rmax enters the
+    bend (then the accumulator restarts), Data/energyChangeAfter.dat at the
+    end; each line x - rmean_x and -statModulus dt times the accumulated
+    field, six digits.  Expected from the library over the same run."""
+    if not os.path.exists(EXE):
+        pytest.skip("driver not built")
+    R, P, Q = gaussian_bunch(2000, (0.0455, 0.0064), 0.4e-3, 0.4e-3, 100.0, 1.0, seed=19, energy_spread=1e-3)
+    steps, stat = 40, 2
+    run_driver(tmp_path, R, P, Q, ["--numsteps", str(steps), "--statmodulus", str(stat)])
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lengthbefore"], length=GEOM["length"],
+                   width=GEOM["width"], phi=GEOM["phi"], Ekin=GEOM["Ekin"], Bz=GEOM["Bz"], order=1,
+                   statmodulus=stat)
+    g = TwoGridStep(cmd)
+    # Bend.hh:307-323 on the entrance plane x = lengthbefore, on the beam axis
+    assert not g.isInsideBend((GEOM["lengthbefore"] - 1e-4, 0.5 * GEOM["width"]))
+    assert g.isInsideBend((GEOM["lengthbefore"] + 1e-4, 0.5 * GEOM["width"]))
+    g.init_bunch(R, P, Q)
+    dt = g.info()["dt"]
+
+    def wake_text(rmean_x):
+        w = g.wake()
+        o, h = g.info()["origin"][0], g.hr[0]
+        return "".join("%g\t%g\n" % (o + (k + 2) * h - rmean_x, -stat * dt * w[k]) for k in range(len(w)))
+
+    want, entered = {}, None
+    for it in range(steps):
+        g.updateField_firstStep()
+        g.updateField_secondStep()
+        if it % stat == stat - 1:
+            s = g.calcBeamParameters()
+            if entered is None and g.isInsideBend((s["rmax_x"], s["rmax_y"])):
+                want["energyChangeBefore.dat"] = wake_text(s["rmean_x"])
+                g.wake_reset()
+                entered = it
+        g.moveMesh()
+    s = g.calcBeamParameters()
+    want["energyChangeAfter.dat"] = wake_text(s["rmean_x"])
+    print("bunch entered the bend at step", entered, flush=True)
+    assert entered is not None and entered > 2
+    for name, text in want.items():
+        got = open(tmp_path / "Data" / name).read()
+        assert got == text, name
+    assert not os.path.exists(tmp_path / "Data" / "energyChangeInBend.dat")
