@@ -96,6 +96,32 @@ def parity_mode_gpu(steps=200, warmup=10):
                          % (inf["W"], inf["H"]))
 
 
+UNET_LEVELS = [0, 0, 1, 1, 2, 2, 3, 3, 3, 2, 2, 1, 1, 0, 0]   # resolution level of each 3x3 convolution
+
+
+def warmstart_bench(H=1024, W=2048, B=4, warmup=2, reps=5):
+    """Inference of the BF16-MFMA warm-start U-Net on the 2048 x 1024 charge
+    densities of the C4 workload, batch B: FLOPs of the 15 convolutions
+    (2 * 9 * cin * cout per output pixel) and wall time per batch including
+    the host copies of the input and output."""
+    import numpy as np
+    from csrst_amd.warmstart import WarmStart, layers
+    L, _ = layers()
+    per_layer = [2 * 9 * l["cin"] * l["cout"] * (H >> lv) * (W >> lv) * B for l, lv in zip(L, UNET_LEVELS)]
+    u = WarmStart(H, W, max_batch=B)
+    rho = np.random.default_rng(1).normal(size=(B, H, W)).astype(np.float32)
+    for _ in range(warmup):
+        u.infer(rho)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        u.infer(rho)
+    t = (time.perf_counter() - t0) / reps
+    u.close()
+    return dict(grid="%dx%d" % (W, H), batch=B, reps=reps, flops_per_batch=float(sum(per_layer)),
+                conv_flops_by_layer=per_layer, wall_ms_per_batch=1e3 * t,
+                wall_tflops=sum(per_layer) / t / 1e12)
+
+
 def _oracle_worker(seconds):
     """Child process of cpu_baseline's all-cores leg: one sample, as JSON."""
     print(json.dumps(_oracle_sample(seconds)), flush=True)
@@ -190,6 +216,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
+    ap.add_argument("--warmstart", action="store_true",
+                    help="only time the warm-start U-Net inference on 2048x1024 densities and print it")
     ap.add_argument("--parity-mode", action="store_true",
                     help="only time the two-grid FP64 step on the CPU baseline's workload and print it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -210,6 +238,9 @@ def main():
         copy_probe(a.copy_probe)
     if a.parity_mode:
         print(json.dumps(parity_mode_gpu(steps=a.steps, warmup=a.warmup)), flush=True)
+        return
+    if a.warmstart:
+        print(json.dumps(warmstart_bench()), flush=True)
         return
     cfg = config(a.config)
     n = a.particles or cfg["n"]

@@ -68,19 +68,23 @@ struct ConvArgs {
     int relu;
 };
 
-// Implicit-GEMM 3x3 convolution.  grid = (ceil(P / 256), CoutP / 16);
-// each of the 4 waves covers 64 consecutive pixels of the block's 256.
+// Implicit-GEMM 3x3 convolution.  grid = (ceil(P / 256), CoutP / (16 MT));
+// each of the 4 waves covers 64 consecutive pixels of the block's 256 and MT
+// tiles of 16 output channels, so each input load feeds MT MFMAs.
+template <int MT>
 __global__ void __launch_bounds__(kB) k_conv(ConvArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = a.B * a.H * a.W;
-    const int co0 = blockIdx.y * 16;
+    const int co0 = blockIdx.y * 16 * MT;
     const int pw = blockIdx.x * 256 + wave * 64;
     if (pw >= P) return;
     const int li = lane & 15, kg = lane >> 4;
-    f4 acc[4];
+    f4 acc[MT][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = f4{0.f, 0.f, 0.f, 0.f};
     // pixel coordinates of this lane's B columns
     int pb[4], py[4], px[4];
     bool pv[4];
@@ -105,17 +109,23 @@ __global__ void __launch_bounds__(kB) k_conv(ConvArgs a)
                 off[t] = ok ? ((pb[t] * a.H + ys) * a.W + xs) * Cin : -1;
             }
             const bf16 *wr = a.w + ((size_t)(co0 + li) * 9 + tap) * Cin;
+            const size_t mstride = (size_t)16 * 9 * Cin;   // next 16 output channels
             for (int c0 = 0; c0 < Cin; c0 += 16) {
-                const sh4 av = load4(wr + c0 + 4 * kg);
+                sh4 av[MT], bv[4];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) av[m] = load4(wr + m * mstride + c0 + 4 * kg);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    sh4 bv = sh4{0, 0, 0, 0};
-                    if (off[t] >= 0) bv = load4(a.x + off[t] + c0 + 4 * kg);
-                    acc[t] = mfma(av, bv, acc[t]);
+                    bv[t] = sh4{0, 0, 0, 0};
+                    if (off[t] >= 0) bv[t] = load4(a.x + off[t] + c0 + 4 * kg);
                 }
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) acc[m][t] = mfma(av[m], bv[t], acc[m][t]);
             }
         }
-    } else {
+    } else if (MT == 1) {
         // small Cin: K = 9 Cin flattened as k = tap * Cin + ci, padded to 16
         const int K = 9 * Cin;
         for (int k0 = 0; k0 < K; k0 += 16) {
@@ -141,25 +151,39 @@ __global__ void __launch_bounds__(kB) k_conv(ConvArgs a)
                     }
                     bv[r] = v;
                 }
-                acc[t] = mfma(av, bv, acc[t]);
+                acc[0][t] = mfma(av, bv, acc[0][t]);
             }
         }
     }
-    // epilogue: rows co0 + 4 kg + r, column = pixel
+    // epilogue: rows co0 + 16 m + 4 kg + r, column = pixel
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        if (!pv[t]) continue;
-        const size_t p = (size_t)pw + 16 * t + li;
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int co = co0 + 4 * kg + r;
-            if (co >= a.Cout) continue;
-            float v = acc[t][r] + (a.bias ? a.bias[co] : 0.f);
-            if (a.relu) v = v > 0.f ? v : 0.f;
-            if (a.y) a.y[p * a.Cout + co] = f2bf(v);
-            if (a.yf) a.yf[p * a.Cout + co] = v;
+        for (int t = 0; t < 4; ++t) {
+            if (!pv[t]) continue;
+            const size_t p = (size_t)pw + 16 * t + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + 16 * m + 4 * kg + r;
+                if (co >= a.Cout) continue;
+                float v = acc[m][t][r] + (a.bias ? a.bias[co] : 0.f);
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.y) a.y[p * a.Cout + co] = f2bf(v);
+                if (a.yf) a.yf[p * a.Cout + co] = v;
+            }
         }
-    }
+}
+
+// k_conv with the widest channel blocking that divides CoutP (small Cin
+// takes the flattened path, one tile)
+void launch_conv(const ConvArgs &a, hipStream_t s)
+{
+    const int P = a.B * a.H * a.W;
+    const int mt = (a.Cin & 15) ? 1 : (a.CoutP % 64 == 0) ? 4 : (a.CoutP % 32 == 0) ? 2 : 1;
+    const dim3 grid((P + 255) / 256, a.CoutP / (16 * mt));
+    if (mt == 4) k_conv<4><<<grid, kB, 0, s>>>(a);
+    else if (mt == 2) k_conv<2><<<grid, kB, 0, s>>>(a);
+    else k_conv<1><<<grid, kB, 0, s>>>(a);
 }
 
 // Weight gradient dW[co][tap][ci] += sum_p dY[p][co] X[p + s(tap)][ci]:
@@ -426,8 +450,7 @@ void conv(csrst_warmstart *u, int li, const bf16 *x, bf16 *y, float *yf, int B, 
     ConvArgs a;
     a.x = x; a.w = u->wb.p + u->wbOff[li]; a.bias = u->w.p + l.boff; a.y = y; a.yf = yf;
     a.B = B; a.H = H; a.W = W; a.Cin = l.cin; a.Cout = l.cout; a.CoutP = pad16(l.cout); a.flip = 0; a.relu = l.relu;
-    const int P = B * H * W;
-    k_conv<<<dim3((P + 255) / 256, a.CoutP / 16), kB, 0, u->s>>>(a);
+    launch_conv(a, u->s);
 }
 
 // input gradient of conv li: dx[P][cin] from dpre[P][cout]
@@ -437,8 +460,7 @@ void conv_dgrad(csrst_warmstart *u, int li, const bf16 *dpre, bf16 *dx, int B, i
     ConvArgs a;
     a.x = dpre; a.w = u->wt.p + u->wtOff[li]; a.bias = nullptr; a.y = dx; a.yf = nullptr;
     a.B = B; a.H = H; a.W = W; a.Cin = l.cout; a.Cout = l.cin; a.CoutP = pad16(l.cin); a.flip = 1; a.relu = 0;
-    const int P = B * H * W;
-    k_conv<<<dim3((P + 255) / 256, a.CoutP / 16), kB, 0, u->s>>>(a);
+    launch_conv(a, u->s);
 }
 
 void conv_wgrad(csrst_warmstart *u, int li, const bf16 *dpre, const bf16 *x, int B, int H, int W)
