@@ -116,10 +116,19 @@ def warmstart_bench(H=1024, W=2048, B=4, warmup=2, reps=5):
     for _ in range(reps):
         u.infer(rho)
     t = (time.perf_counter() - t0) / reps
+    # training: forward, input gradients and weight gradients (about three
+    # times the forward FLOPs) and the Adam update
+    phi = np.random.default_rng(2).normal(size=(B, H, W)).astype(np.float32)
+    u.train_step(rho, phi, lr=1e-4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        u.train_step(rho, phi, lr=1e-4)
+    tt = (time.perf_counter() - t0) / reps
     u.close()
     return dict(grid="%dx%d" % (W, H), batch=B, reps=reps, flops_per_batch=float(sum(per_layer)),
                 conv_flops_by_layer=per_layer, wall_ms_per_batch=1e3 * t,
-                wall_tflops=sum(per_layer) / t / 1e12)
+                wall_tflops=sum(per_layer) / t / 1e12, train_ms_per_batch=1e3 * tt,
+                train_pairs_per_s=B / tt, train_wall_tflops=3 * sum(per_layer) / tt / 1e12)
 
 
 def _oracle_worker(seconds):
