@@ -247,6 +247,77 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(const bf16 *dy, const bf16 *x
     }
 }
 
+// Weight gradient for Cin and Cout multiples of 16: per step of 16 pixels
+// the dY tile (16 pixels x 16 output channels) and the four shifted X tiles
+// (16 pixels x 16 input channels each; a tile of 16 n = tap Cin + ci lies in
+// one tap) are read with 8-byte loads (4 channels of one pixel per lane),
+// staged in LDS and read back in the MFMA operand layout.  Same grid and
+// results as k_conv_wgrad up to the order of the FP32 sums.
+__global__ void __launch_bounds__(64) k_conv_wgrad16(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
+                                                    int Cin, int Cout, int chunk)
+{
+    __shared__ bf16 sA[16][16 + 4];
+    __shared__ bf16 sB[4][16][16 + 4];
+    const int lane = threadIdx.x, li = lane & 15, kg = lane >> 4;
+    const int P = B * H * W, N = 9 * Cin;
+    const int co0 = blockIdx.y * 16;
+    const int n0 = blockIdx.x * 64;
+    const int k0 = blockIdx.z * chunk, k1 = min(P, k0 + chunk);
+    const int lp = lane >> 2, lc = (lane & 3) * 4;   // loader: pixel and 4 channels
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    int tapt[4], ci0[4];
+    bool tv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int n = n0 + 16 * t;
+        tv[t] = n < N;
+        tapt[t] = tv[t] ? n / Cin : 0;
+        ci0[t] = tv[t] ? n % Cin : 0;
+    }
+    for (int kb = k0; kb < k1; kb += 16) {
+        const int p = kb + lp;
+        uint2 va = make_uint2(0u, 0u);
+        uint2 vb[4] = {make_uint2(0u, 0u), make_uint2(0u, 0u), make_uint2(0u, 0u), make_uint2(0u, 0u)};
+        if (p < k1) {
+            va = *reinterpret_cast<const uint2 *>(dy + (size_t)p * Cout + co0 + lc);
+            const int xx = p % W, yy = (p / W) % H, bb = p / (W * H);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int ys = yy + tapt[t] / 3 - 1, xs = xx + tapt[t] % 3 - 1;
+                if (tv[t] && ys >= 0 && ys < H && xs >= 0 && xs < W)
+                    vb[t] = *reinterpret_cast<const uint2 *>(x + ((size_t)(bb * H + ys) * W + xs) * Cin + ci0[t] + lc);
+            }
+        }
+        __syncthreads();
+        *reinterpret_cast<uint2 *>(&sA[lp][lc]) = va;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) *reinterpret_cast<uint2 *>(&sB[t][lp][lc]) = vb[t];
+        __syncthreads();
+        sh4 av;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) av[r] = (short)sA[4 * kg + r][li];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            sh4 bv;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bv[r] = (short)sB[t][4 * kg + r][li];
+            acc[t] = mfma(av, bv, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (!tv[t]) continue;
+        const int n = n0 + 16 * t + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = co0 + 4 * kg + r;
+            atomicAdd(&dw[(size_t)c * N + n], acc[t][r]);
+        }
+    }
+}
+
 // bias gradient: db[co] += sum_p dY[p][co]
 __global__ void k_bias_grad(const bf16 *dy, float *db, int P, int C)
 {
@@ -469,7 +540,10 @@ void conv_wgrad(csrst_warmstart *u, int li, const bf16 *dpre, const bf16 *x, int
     const int P = B * H * W;
     const int chunk = 1024;
     dim3 grid((9 * l.cin + 63) / 64, pad16(l.cout) / 16, (P + chunk - 1) / chunk);
-    k_conv_wgrad<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
+    if (l.cin % 16 == 0 && l.cout % 16 == 0)
+        k_conv_wgrad16<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
+    else
+        k_conv_wgrad<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
     dim3 bg(std::min(64, (P + kB - 1) / kB), l.cout);
     k_bias_grad<<<bg, kB, 0, u->s>>>(dpre, u->g.p + l.boff, P, l.cout);
 }
