@@ -6,7 +6,7 @@
  * product path.  Only tests/, the smoke() checker and the CPU-baseline leg of
  * bench.py may load it.  The product library never links, loads or calls it.
  *
- * Build: g++ -O2 -ffp-contract=off (no FMA contraction, like the reference's
+ * Build: g++ -O3 -ffp-contract=off (no FMA contraction, like the reference's
  * ISO C++ build, CMakeLists.txt:85-87).
  *
  * Pinning status of each function is listed in oracle/README.md.

@@ -35,7 +35,9 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # CSRST_ORACLE_LIB loads another build of the same source (the mutation
+        # tests use it for deliberately broken copies)
+        path = os.environ.get("CSRST_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
