@@ -1,10 +1,14 @@
 """Benchmark of the CSRST PIC time step on one MI355X.
 
-Workload (default): 8192 x 2048 window at 10 um, 30 deg bend of radius 5 m,
-seeded Gaussian bunch gamma 500, 0.5 nC, sigma_s 200 um, sigma_x 100 um,
-energy spread 1e-3, 1e7 macro-particles, second-order shape function, cell
-sort every 20 steps.  The FP32 split-precision step is timed, then the same
-run in FP64; the CSR wake of both runs is compared.
+Workload (default, c3): 32768 x 4096 window at 5 um, 20 deg bend of radius
+10 m, seeded Gaussian bunch gamma 1000, 1 nC, sigma_s = sigma_x = 50 um,
+linear energy chirp -10/m, 1e8 macro-particles, second-order shape
+function, cell sort every 20 steps.  The timed slice starts with the bunch
+half-way round the bend (10 deg into the arc, moving along its tangent, the
+bend field acting on every particle).  The FP32 split-precision step is
+timed there, then the same run in FP64, and the CSR wake of both runs is
+compared; the FP32 step with the bunch 5 mm before the bend (moving along
+the beam axis) is timed beside it.
 
 Prints one JSON line (the benchmark record).  A step = one full TE/TM time
 step (two pushes with deposits, the field sweeps, the window move).
@@ -218,12 +222,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--slice", default="bend", choices=("bend", "straight"),
+                    help="bend: bunch half-way round the arc (headline); straight: 5 mm before the bend")
     ap.add_argument("--particles", type=int, default=0, help="override the particle count")
     ap.add_argument("--angle", type=float, default=0.0,
                     help="bunch direction in degrees from the beam axis (a bend's later part); not the headline")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-straight", action="store_true", help="skip the straight-section slice")
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
     ap.add_argument("--warmstart", action="store_true",
                     help="only time the warm-start U-Net inference on 2048x1024 densities and print it")
@@ -239,7 +246,7 @@ def main():
 
     import numpy as np
     from csrst_amd import _lib
-    from csrst_amd.configs import config, bunch_for
+    from csrst_amd.configs import config, bunch_for, bunch_in_bend
     if not _lib.has_gpu():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
@@ -253,9 +260,12 @@ def main():
         return
     cfg = config(a.config)
     n = a.particles or cfg["n"]
-    print("bench: config %s, %d particles, %dx%d cells, %d+%d steps" % (a.config, n, cfg["W"], cfg["H"], a.warmup,
-                                                                        a.steps), file=sys.stderr, flush=True)
-    R, P, Q = bunch_for(cfg, n=n, seed=1, angle_deg=a.angle)
+    print("bench: config %s, %d particles, %dx%d cells, %d+%d steps, slice %s" % (
+        a.config, n, cfg["W"], cfg["H"], a.warmup, a.steps, a.slice), file=sys.stderr, flush=True)
+    if a.slice == "bend" and not a.angle:
+        R, P, Q = bunch_in_bend(cfg, n=n, seed=1, arc_fraction=0.5)
+    else:
+        R, P, Q = bunch_for(cfg, n=n, seed=1, angle_deg=a.angle)
     t32, tim32, wake32, info32 = run_mw(cfg, True, a.steps, a.warmup, R, P, Q, a.stat)
     print("bench: fp32 %.3f s, timing %s" % (t32, tim32), file=sys.stderr, flush=True)
     cells = cfg["W"] * cfg["H"]
@@ -270,16 +280,30 @@ def main():
         rec64 = dict(particle_steps_per_s=n * a.steps / t64, cell_updates_per_s=cells * a.steps / t64,
                      ms_per_step=1e3 * t64 / a.steps, speedup_fp32_over_fp64=t64 / t32, wake_rel_err=werr,
                      window_moves_match=info64["pos"] == info32["pos"], timing_ms=tim64)
+    other = None
+    if a.slice == "bend" and not a.angle and not a.no_straight:
+        del R, P, Q
+        Rs, Ps, Qs = bunch_for(cfg, n=n, seed=1)
+        ts, tims, _, _ = run_mw(cfg, True, a.steps, a.warmup, Rs, Ps, Qs, a.stat)
+        del Rs, Ps, Qs
+        print("bench: fp32 straight %.3f s, timing %s" % (ts, tims), file=sys.stderr, flush=True)
+        other = dict(slice="straight: bunch centred 5 mm before the bend entry, moving along the beam axis",
+                     particle_steps_per_s=n * a.steps / ts, cell_updates_per_s=cells * a.steps / ts,
+                     ms_per_step=1e3 * ts / a.steps, push_launch_ms=tims["push_ms"] / 2.0,
+                     sweep_ms_per_step=tims["sweep_ms"])
     # roofline of the dominant kernel (the particle push + deposit, two launches per step)
     push_launch_ms = tim32["push_ms"] / 2.0
     sweep_ms = tim32["sweep_ms"]
     achieved = n * B_PART[True] / (push_launch_ms * 1e-3) / 1e9
     frac_step = (cells * B_CELL[True] + n * B_PART[True]) * a.steps / t32 / 1e9 / HBM_PEAK_GBS
+    frac_step48 = (cells * 48 + n * B_PART[True]) * a.steps / t32 / 1e9 / HBM_PEAK_GBS
     workload = ("%s: %dx%d window dx=%g um, %.0f deg bend R=%g m, gamma %g, %g nC, %d particles, shape S%d, "
                 "sort every %d" % (a.config, cfg["W"], cfg["H"], cfg["dx"] * 1e6, cfg["phiDeg"], cfg["R"],
                                    cfg["gamma"], cfg["charge_nC"], n, cfg["order"], cfg["sortEvery"]))
     if a.angle:
         workload += ", bunch at %g deg" % a.angle
+    elif a.slice == "bend":
+        workload += ", bunch half-way round the bend"
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()
@@ -300,6 +324,10 @@ def main():
         "config": {"workload": workload, "cells": cells, "particles": n, "pml": cfg["pml"], "cfl": cfg["cfl"]},
         "cell_updates_per_s": cu32,
         "step_roofline_frac": frac_step,
+        "step_roofline_frac_48B_cells": frac_step48,
+        "slice": "bend: bunch centred half-way round the arc, moving along its tangent" if a.slice == "bend"
+                 else "straight: bunch centred 5 mm before the bend entry",
+        "other_slice": other,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "k_push (push + deposit, %d B per particle per launch)" % B_PART[True],

@@ -18,17 +18,21 @@ ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EWINDOW", -5: "EGEOM", -6
 # every symbol declared in include/csrst_hip.h
 SYMBOLS = [
     "csrst_version", "csrst_last_error", "csrst_device_count",
-    "csrst_create", "csrst_destroy", "csrst_geometry", "csrst_boundary_cells", "csrst_init_bunch",
+    "csrst_create", "csrst_destroy", "csrst_geometry", "csrst_boundary_cells", "csrst_set_mask",
+    "csrst_set_boundary_cells", "csrst_set_stream", "csrst_init_bunch",
     "csrst_restore_bunch",
     "csrst_step", "csrst_first_step", "csrst_second_step", "csrst_move_window", "csrst_copy_fields", "csrst_substep",
     "csrst_download_field", "csrst_upload_field", "csrst_download_particles", "csrst_download_charges",
     "csrst_info",
     "csrst_wake", "csrst_wake_reset", "csrst_is_inside_bend", "csrst_diag", "csrst_beam_stats", "csrst_field_energy",
-    "csrst_mw_create", "csrst_mw_destroy", "csrst_mw_init_bunch", "csrst_mw_step", "csrst_mw_place",
+    "csrst_mw_create", "csrst_mw_destroy", "csrst_mw_init_bunch", "csrst_mw_step", "csrst_mw_push", "csrst_mw_set_stream",
+    "csrst_mw_place",
     "csrst_mw_upload_fields",
     "csrst_mw_download_fields", "csrst_mw_download_particles", "csrst_mw_beam_stats", "csrst_mw_info", "csrst_mw_wake",
     "csrst_mw_timing",
     "csrst_poisson_solve", "csrst_poisson_compute_field", "csrst_mw_poisson",
+    "csrst_sg_gather", "csrst_sg_scatter", "csrst_sf_current_density", "csrst_sf_charge_density",
+    "csrst_sf_charge_density_diff",
     "csrst_warmstart_create", "csrst_warmstart_destroy", "csrst_warmstart_num_params",
     "csrst_warmstart_get_weights", "csrst_warmstart_set_weights", "csrst_warmstart_infer",
     "csrst_warmstart_gradients", "csrst_warmstart_train_step",
@@ -52,7 +56,7 @@ class MWParams(C.Structure):
                 ("lenBefore", C.c_double), ("lenAfter", C.c_double), ("width", C.c_double),
                 ("phiDeg", C.c_double), ("ekinMeV", C.c_double), ("bz", C.c_double),
                 ("order", C.c_int), ("precision", C.c_int), ("sortEvery", C.c_int),
-                ("statModulus", C.c_int), ("cfl", C.c_double), ("pml", C.c_int)]
+                ("statModulus", C.c_int), ("cfl", C.c_double), ("pml", C.c_int), ("wakeGrid", C.c_int)]
 
 
 def lib():

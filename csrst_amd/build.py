@@ -13,11 +13,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcsrst.so")
-SOURCES = ["csrst_ref.hip", "csrst_mw.hip", "csrst_poisson.hip", "csrst_unet.hip"]
-# The two-grid step must stay bit-identical to the reference arithmetic, so its
-# translation unit forbids FMA contraction.  The production window, the
+SOURCES = ["csrst_ref.hip", "csrst_mw.hip", "csrst_poisson.hip", "csrst_unet.hip", "csrst_sf.hip"]
+# The two-grid step and the stateless ShapeFunction / ScatterGatherImpl
+# kernels must stay bit-identical to the reference arithmetic, so their
+# translation units forbid FMA contraction.  The production window, the
 # multigrid and the network are tolerance-checked and use fused multiply-adds.
-CONTRACT = {"csrst_ref.hip": "off"}
+CONTRACT = {"csrst_ref.hip": "off", "csrst_sf.hip": "off"}
 ARCH = os.environ.get("CSRST_ARCH", "gfx950")
 
 

@@ -48,6 +48,30 @@ def config(name):
     raise KeyError(name)
 
 
+def arc_point(cfg, into_arc):
+    """Centre line point `into_arc` metres along the bend's arc and the
+    direction of the arc there (the bend of Bend.cpp:60-124: centre of
+    curvature at (lenBefore, width/2 + R) for Bz > 0)."""
+    R = cfg["R"]
+    phi0 = into_arc / R
+    sgn = 1.0 if cfg["bz"] > 0 else -1.0
+    cx, cy = cfg["lenBefore"], 0.5 * cfg["width"] + sgn * R
+    pos = (cx + R * np.sin(phi0), cy - sgn * R * np.cos(phi0))
+    return pos, sgn * phi0
+
+
+def bunch_in_bend(cfg, n=None, seed=1, arc_fraction=0.5):
+    """Seeded bunch of the configuration centred on the bend's centre line at
+    `arc_fraction` of the arc, moving along the arc's tangent there."""
+    from .bunch import gaussian_bunch
+    n = cfg["n"] if n is None else n
+    arc = np.radians(cfg["phiDeg"]) * cfg["R"]
+    centre, th = arc_point(cfg, arc_fraction * arc)
+    return gaussian_bunch(n, centre, cfg["sigma_s"], cfg["sigma_x"], cfg["gamma"], cfg["charge_nC"], seed=seed,
+                          direction=(float(np.cos(th)), float(np.sin(th))), energy_spread=cfg["espread"],
+                          chirp=cfg["chirp"], modulation=cfg.get("modulation"))
+
+
 def bunch_for(cfg, n=None, seed=1, s_offset=-5e-3, angle_deg=0.0):
     """Seeded bunch of the configuration, centred s_offset before the bend entry, mid-chamber,
     moving at angle_deg to the beam axis (0: along it)."""

@@ -1358,18 +1358,22 @@ __global__ void __launch_bounds__(kB) k_moments_final(const DD *partial, int n, 
 }
 
 // Wake: E along the line through the centroid in the direction of the mean
-// momentum, at ns positions spaced h0 centred on the centroid, with the
-// rotation convention of IntegrateElong.cpp:49-61; accumulated with
-// compensation over the sampling steps.
+// momentum, with the rotation convention of IntegrateElong.cpp:49-61;
+// accumulated with compensation over the sampling steps.  Sample k sits at
+//  - grid = 0: the distance (k - ns/2) h0 from the centroid (a fixed set of
+//    offsets along the bunch);
+//  - grid = 1: the distance X - x_centroid, X the x of the window's grid
+//    column k + 2, as IntegrateElong.cpp:59-61 takes the window's grid
+//    points (the offsets shift with the window).
 template <class T>
-__global__ void k_wake(Fields<T> f, Geo g, const WinState *ws, const double *mom, int ns, DD *acc)
+__global__ void k_wake(Fields<T> f, Geo g, const WinState *ws, const double *mom, int ns, DD *acc, int grid)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     const double R0 = mom[0], P0 = mom[1], R1 = mom[2], P1 = mom[3];
     const double lenP = sqrt(P0 * P0 + P1 * P1);
     const double cosP = P0 / lenP, sinP = P1 / lenP;
-    const double l0 = (double)(k - ns / 2) * g.h[0];
+    const double l0 = grid ? (g.org[0] + (double)(ws->wx0 + k + 2) * g.h[0]) - R0 : (double)(k - ns / 2) * g.h[0];
     const double x = (cosP * l0) + R0;
     const double y = (-sinP * l0) + R1;
     // position in global cells
@@ -1479,7 +1483,8 @@ struct csrst_mw {
     csrst_mw_params p{};
     mw::Geo geo{};
     double dt = 0;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;     // the stream the step's work goes to
+    hipStream_t own = nullptr;   // the step's own stream (s unless the caller set one)
     int W = 0, H = 0, Wp = 0, Hp = 0;
     size_t plane = 0;
     bool f32 = true;
@@ -1644,7 +1649,7 @@ void wake(csrst_mw *m)
     mw::k_moments<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->geo, m->ws.p, m->partial.p);
     mw::k_moments_final<<<1, mw::kB, 0, m->s>>>(m->partial.p, m->n, m->mom.p);
     mw::k_wake<T><<<(m->nsWake + mw::kB - 1) / mw::kB, mw::kB, 0, m->s>>>(m->fields<T>(), m->geo, m->ws.p, m->mom.p,
-                                                                          m->nsWake, m->wacc.p);
+                                                                          m->nsWake, m->wacc.p, m->p.wakeGrid);
     HU_LAUNCH_CHECK();
 }
 
@@ -1813,6 +1818,7 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         m->p = *p;
         m->f32 = p->precision == 1;
         HU_CHECK(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
+        m->own = m->s;
         for (auto &e : m->ev) HU_CHECK(hipEventCreate(&e));
         // geometry: the bend of Bend.cpp with the configured spacing; the
         // virtual global mesh origin as in Bend::Init (vertex 1 on x = 0)
@@ -1884,7 +1890,8 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
 int csrst_mw_destroy(csrst_mw *m)
 {
     if (!m) return CSRST_OK;
-    if (m->s) { (void)hipStreamSynchronize(m->s); (void)hipStreamDestroy(m->s); }
+    if (m->s) (void)hipStreamSynchronize(m->s);
+    if (m->own) (void)hipStreamDestroy(m->own);
     for (auto &e : m->ev) if (e) (void)hipEventDestroy(e);
     delete m;
     return CSRST_OK;
@@ -2016,6 +2023,32 @@ int csrst_mw_step(csrst_mw *m, int nsteps)
             acc[3] += t67;
         }
         for (int k = 0; k < 4; ++k) m->lastMs[k] = nsteps ? acc[k] / nsteps : 0;
+        check_state(m);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_set_stream(csrst_mw *m, void *stream)
+{
+    try {
+        if (!m) throw Error(CSRST_EINVAL, "null");
+        HU_CHECK(hipStreamSynchronize(m->s));
+        m->s = stream ? (hipStream_t)stream : m->own;
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return mw_fail(m, e);
+    }
+}
+
+int csrst_mw_push(csrst_mw *m, int comp)
+{
+    try {
+        if (!m || !m->ready || m->n <= 0) throw Error(CSRST_ESTATE, "csrst_mw_push before csrst_mw_init_bunch");
+        if (comp != 0 && comp != 1) throw Error(CSRST_EINVAL, "comp must be 0 (Jx) or 1 (Jy)");
+        if (m->f32) { if (comp == 0) push_dispatch<float, 0>(m); else push_dispatch<float, 1>(m); }
+        else { if (comp == 0) push_dispatch<double, 0>(m); else push_dispatch<double, 1>(m); }
         check_state(m);
         return CSRST_OK;
     } catch (const std::exception &e) {

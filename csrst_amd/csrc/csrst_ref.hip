@@ -755,7 +755,8 @@ struct csrst_ctx {
     geom::Bend bend{};
     Margins mg{};
     double dt = 0, hr[2] = {0, 0};
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;     // the stream the context's work goes to
+    hipStream_t own = nullptr;   // the context's own stream (s unless the caller set one)
     // fields
     DBuf<double> gE, gH, gJ, mEb, mHb, mJb, snapb;
     FieldSet fs{}, snap{};
@@ -806,6 +807,34 @@ struct csrst_ctx {
 };
 
 namespace {
+
+// The mask and the boundary-cell list (host copies in c) to the device.
+void upload_geometry(csrst_ctx *c)
+{
+    c->mask.alloc(c->hmask.size());
+    HU_CHECK(hipMemcpyAsync(c->mask.p, c->hmask.data(), c->hmask.size(), hipMemcpyHostToDevice, c->s));
+    const int nb = (int)(c->hbij.size() / 2);
+    if (nb > 0) {
+        std::vector<int> vi(nb), vj(nb);
+        std::vector<double> l0(nb), l1(nb), ar(nb), n0(nb), n1(nb);
+        for (int k = 0; k < nb; ++k) {
+            vi[k] = c->hbij[2 * k]; vj[k] = c->hbij[2 * k + 1];
+            l0[k] = c->hbdata[5 * k]; l1[k] = c->hbdata[5 * k + 1]; ar[k] = c->hbdata[5 * k + 2];
+            n0[k] = c->hbdata[5 * k + 3]; n1[k] = c->hbdata[5 * k + 4];
+        }
+        c->bi.alloc(nb); c->bj.alloc(nb);
+        c->blam0.alloc(nb); c->blam1.alloc(nb); c->barea.alloc(nb); c->blamN0.alloc(nb); c->blamN1.alloc(nb);
+        HU_CHECK(hipMemcpy(c->bi.p, vi.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->bj.p, vj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->blam0.p, l0.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->blam1.p, l1.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->barea.p, ar.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->blamN0.p, n0.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HU_CHECK(hipMemcpy(c->blamN1.p, n1.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->bc = BCells{c->bi.p, c->bj.p, c->blam0.p, c->blam1.p, c->barea.p, c->blamN0.p, c->blamN1.p, nb};
+    HU_CHECK(hipStreamSynchronize(c->s));
+}
 
 int fail(csrst_ctx *c, const std::exception &e)
 {
@@ -1329,6 +1358,7 @@ int csrst_create(const csrst_params *p, csrst_ctx **out)
         if (p->order < 0 || p->order > 2) throw Error(CSRST_EINVAL, "order must be 0, 1 or 2");
         c->p = *p;
         HU_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        c->own = c->s;
         geom::bend_init(c->bend, p->lenBefore, p->lenAfter, p->width, p->phiDeg, p->ekinMeV, p->bz, p->Nx, p->Ny);
         c->hr[0] = c->bend.hr[0];
         c->hr[1] = c->bend.hr[1];
@@ -1356,28 +1386,7 @@ int csrst_create(const csrst_params *p, csrst_ctx **out)
                     c->hbdata.push_back(cut.lamN[0]); c->hbdata.push_back(cut.lamN[1]);
                 }
             }
-        c->mask.alloc(c->hmask.size());
-        HU_CHECK(hipMemcpyAsync(c->mask.p, c->hmask.data(), c->hmask.size(), hipMemcpyHostToDevice, c->s));
-        const int nb = (int)(c->hbij.size() / 2);
-        if (nb > 0) {
-            std::vector<int> vi(nb), vj(nb);
-            std::vector<double> l0(nb), l1(nb), ar(nb), n0(nb), n1(nb);
-            for (int k = 0; k < nb; ++k) {
-                vi[k] = c->hbij[2 * k]; vj[k] = c->hbij[2 * k + 1];
-                l0[k] = c->hbdata[5 * k]; l1[k] = c->hbdata[5 * k + 1]; ar[k] = c->hbdata[5 * k + 2];
-                n0[k] = c->hbdata[5 * k + 3]; n1[k] = c->hbdata[5 * k + 4];
-            }
-            c->bi.alloc(nb); c->bj.alloc(nb);
-            c->blam0.alloc(nb); c->blam1.alloc(nb); c->barea.alloc(nb); c->blamN0.alloc(nb); c->blamN1.alloc(nb);
-            HU_CHECK(hipMemcpy(c->bi.p, vi.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->bj.p, vj.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->blam0.p, l0.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->blam1.p, l1.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->barea.p, ar.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->blamN0.p, n0.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-            HU_CHECK(hipMemcpy(c->blamN1.p, n1.data(), nb * sizeof(double), hipMemcpyHostToDevice));
-        }
-        c->bc = BCells{c->bi.p, c->bj.p, c->blam0.p, c->blam1.p, c->barea.p, c->blamN0.p, c->blamN1.p, nb};
+        upload_geometry(c.get());
         c->bb.alloc(2);
         c->derr.alloc(1);
         c->derr.zero(c->s);
@@ -1394,9 +1403,61 @@ int csrst_create(const csrst_params *p, csrst_ctx **out)
 int csrst_destroy(csrst_ctx *c)
 {
     if (!c) return CSRST_OK;
-    if (c->s) { (void)hipStreamSynchronize(c->s); (void)hipStreamDestroy(c->s); }
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return CSRST_OK;
+}
+
+int csrst_set_stream(csrst_ctx *c, void *stream)
+{
+    try {
+        if (!c) throw Error(CSRST_EINVAL, "null context");
+        HU_CHECK(hipStreamSynchronize(c->s));
+        c->s = stream ? (hipStream_t)stream : c->own;
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_set_mask(csrst_ctx *c, const uint8_t *mask)
+{
+    try {
+        if (!c || !mask) throw Error(CSRST_EINVAL, "null argument");
+        if (c->ready) throw Error(CSRST_ESTATE, "the geometry is set before csrst_init_bunch");
+        const size_t nn = (size_t)c->p.Nx * c->p.Ny;
+        for (size_t k = 0; k < nn; ++k)
+            if (mask[k] > 1) throw Error(CSRST_EINVAL, "mask values must be 0 or 1");
+        c->hmask.assign(mask, mask + nn);
+        upload_geometry(c);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_set_boundary_cells(csrst_ctx *c, const int *ij, const double *data, int count)
+{
+    try {
+        if (!c || count < 0 || (count > 0 && (!ij || !data))) throw Error(CSRST_EINVAL, "bad boundary-cell list");
+        if (c->ready) throw Error(CSRST_ESTATE, "the geometry is set before csrst_init_bunch");
+        const int Nx = c->p.Nx, Ny = c->p.Ny;
+        std::vector<uint8_t> seen((size_t)Nx * Ny, 0);
+        for (int k = 0; k < count; ++k) {
+            const int i = ij[2 * k], j = ij[2 * k + 1];
+            // the updates read E(i + 1, j), E(i, j + 1), H(i - 1, j), H(i, j - 1): interior cells only
+            if (i < 0 || i >= Nx || j < 0 || j >= Ny) throw Error(CSRST_EINVAL, hu::fmt("cell %d outside the grid", k));
+            if (seen[(size_t)j * Nx + i]++) throw Error(CSRST_EINVAL, hu::fmt("cell (%d, %d) listed twice", i, j));
+            if (!(data[5 * k + 2] > 0)) throw Error(CSRST_EINVAL, hu::fmt("cell %d: area must be positive", k));
+        }
+        c->hbij.assign(ij, ij + 2 * (size_t)count);
+        c->hbdata.assign(data, data + 5 * (size_t)count);
+        upload_geometry(c);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
 }
 
 int csrst_geometry(csrst_ctx *c, double *o)

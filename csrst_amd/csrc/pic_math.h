@@ -35,6 +35,11 @@ PM_HD vec2<T> mk(T a, T b) { vec2<T> r; r.x = a; r.y = b; return r; }
 // (1.5 and the Kadlec pair 1.5010936 / 0.5010936).
 PM_HD float q_rsqrt(float x)
 {
+#if defined(__clang__)
+    // as the reference's ISO C++ build: no fused multiply-adds, so the result
+    // is the reference's float bit for bit in every translation unit
+#pragma clang fp contract(off)
+#endif
     const float xhalf = 0.5f * x;
     const float xqhalf = 0.5010936f * x;
     uint32_t i = __builtin_bit_cast(uint32_t, x);

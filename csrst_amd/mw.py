@@ -15,10 +15,10 @@ C_LIGHT = 299792458.0
 
 class MovingWindowStep:
     def __init__(self, W, H, dx, dy, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order=2, fp32=True,
-                 sortEvery=20, statModulus=100, cfl=1.0, pml=0):
+                 sortEvery=20, statModulus=100, cfl=1.0, pml=0, wakeGrid=False):
         self.W, self.H = W, H
         self._p = _lib.MWParams(W, H, dx, dy, lenBefore, lenAfter, width, phiDeg, ekinMeV, bz, order,
-                                1 if fp32 else 0, sortEvery, statModulus, cfl, pml)
+                                1 if fp32 else 0, sortEvery, statModulus, cfl, pml, int(wakeGrid))
         h = C.c_void_p()
         check(_lib.lib().csrst_mw_create(C.byref(self._p), C.byref(h)))
         self.h = h
@@ -54,6 +54,11 @@ class MovingWindowStep:
 
     def step(self, n=1):
         check(_lib.lib().csrst_mw_step(self.h, C.c_int(n)), self.h)
+
+    def push(self, comp):
+        """One push with the deposit of J[comp] on the current fields, then the
+        drift back by half a step (csrst_mw_push)."""
+        check(_lib.lib().csrst_mw_push(self.h, C.c_int(comp)), self.h)
 
     def timing(self):
         out = np.zeros(5)

@@ -86,6 +86,28 @@ class TwoGridStep:
         check(L.csrst_boundary_cells(self.h, u8ptr(mask), iptr(ij), dptr(data), n, C.byref(cnt)), self.h)
         return mask, ij[:n], data[:n]
 
+    def set_mask(self, mask):
+        """The inside mask [Ny, Nx] of the masked sweeps (before init_bunch)."""
+        m = np.ascontiguousarray(mask, np.uint8)
+        if m.shape != (self.cmd.numgridpy, self.cmd.numgridpx):
+            raise ValueError("mask must be [Ny, Nx]")
+        check(_lib.lib().csrst_set_mask(self.h, u8ptr(m)), self.h)
+
+    def set_boundary_cells(self, ij, data):
+        """SCBoundaryCondition's cell list (SCBoundaryCondition.hh:39-84): ij
+        [n, 2] cell indices, data [n, 5] = lambda0, lambda1, area, lambdaN0,
+        lambdaN1 (before init_bunch)."""
+        ij = np.ascontiguousarray(ij, np.int32).reshape(-1, 2)
+        data = np.ascontiguousarray(data, np.float64).reshape(-1, 5)
+        if len(ij) != len(data):
+            raise ValueError("ij and data differ in length")
+        check(_lib.lib().csrst_set_boundary_cells(self.h, iptr(ij), dptr(data), C.c_int(len(ij))), self.h)
+
+    def set_stream(self, stream):
+        """Queue the step's GPU work on a caller hipStream_t handle (None: the
+        context's own stream)."""
+        check(_lib.lib().csrst_set_stream(self.h, C.c_void_p(stream) if stream else None), self.h)
+
     def init_bunch(self, R, P, Q):
         R = np.ascontiguousarray(R, np.float64).reshape(-1, 2)
         P = np.ascontiguousarray(P, np.float64).reshape(-1, 2)

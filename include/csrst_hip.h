@@ -84,6 +84,20 @@ int csrst_init_bunch(csrst_ctx *c, const double *R, const double *P, const doubl
 int csrst_restore_bunch(csrst_ctx *c, const double *R, const double *oldR, const double *P, const double *Q, int n,
                         int step);
 
+/* Caller geometry in place of the one csrst_create computed from the Bend,
+ * before csrst_init_bunch: the inside mask (Nx*Ny bytes, 0 or 1, the cells
+ * the masked sweeps update, TETMSolver.hh:142-249) and the conformal cell
+ * list of SCBoundaryCondition::getBoundaryCells (SCBoundaryCondition.hh:39-84,
+ * BoundaryCell.hh:28-33): ij 2 ints per cell, data (lambda0, lambda1, area,
+ * lambdaN0, lambdaN1) per cell, areas after limitArea.  The layout is that of
+ * csrst_boundary_cells.  The Poisson start uses the same cells. */
+int csrst_set_mask(csrst_ctx *c, const uint8_t *mask);
+int csrst_set_boundary_cells(csrst_ctx *c, const int *ij, const double *data, int count);
+/* The context's GPU work goes to `stream` (a hipStream_t; NULL: the
+ * context's own stream) from here on; the call first waits for the work
+ * already queued.  The caller keeps the stream alive while it is set. */
+int csrst_set_stream(csrst_ctx *c, void *stream);
+
 /* Whole steps (main.cpp:390-506). */
 int csrst_step(csrst_ctx *c, int nsteps);
 /* Fine-grained parts of one step, for bisection against the CPU oracle. */
@@ -150,6 +164,9 @@ typedef struct {
     double cfl;              /* c*dt/dx; 1.0 is the reference's dt = hx/c */
     int pml;                 /* absorbing-layer columns at each window end (0: none, as the
                                 reference, whose PML coefficients are zeroed, TETMSolver.cpp:59-62) */
+    int wakeGrid;            /* wake samples: 0 at fixed offsets (k - ns/2) dx from the centroid
+                                along the mean momentum; 1 at the window's grid columns k + 2, the
+                                points IntegrateElong.cpp:59-61 takes */
 } csrst_mw_params;
 
 int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out);
@@ -157,6 +174,13 @@ int csrst_mw_destroy(csrst_mw *m);
 /* Positions R in metres (global chamber frame), P = beta*gamma, Q [C]. */
 int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const double *Q, int n);
 int csrst_mw_step(csrst_mw *m, int nsteps);
+/* csrst_set_stream for the moving-window step. */
+int csrst_mw_set_stream(csrst_mw *m, void *stream);
+/* One push of the step on the fields as they stand (PartBunch::push_particles,
+ * PartBunch.cpp:215-519, with its deposit of J[comp], comp 0 = Jx, 1 = Jy,
+ * added into the window's J), followed by the drift by -dt/2 that the step
+ * makes after it (TETMSolver.hh:210, 265).  No field update, no window move. */
+int csrst_mw_push(csrst_mw *m, int comp);
 /* Field-only start: window logical cell (0, 0) at global cell (wx0, wy0),
  * geometry computed, fields zero, no particles (steps then run the field
  * sweeps alone and the window stays in place). */
@@ -184,6 +208,48 @@ int csrst_mw_timing(csrst_mw *m, double *ms, int cap);
  * any window that holds the bunch).  Multigrid in FP64; the fields are stored
  * in the window's precision. */
 int csrst_mw_poisson(csrst_mw *m, double tol, int maxit, int *cycles, double *resid);
+
+/* ---- ShapeFunction and ScatterGatherImpl on caller memory --------------------
+ * Stateless FP64 kernels with the reference's arithmetic, bit-identical to a
+ * CPU loop over the particles in their order (sums over particles are
+ * ordered by a stable sort of the contributions).  Every array is a device
+ * pointer; positions R and oldR are (x, y) pairs per particle.  Fields are
+ * node patches [ny][nx], x fastest.  `stream` is a hipStream_t (NULL: the
+ * null stream); each call returns once its work has finished.  A stencil
+ * that leaves the patch returns CSRST_EWINDOW (its values dropped or read as
+ * zero); the other nodes are still written.
+ *
+ * ScatterGatherImpl<IntOp,double>::gather / ::scatter
+ * (ScatterGatherImpl.hh:27-64; IntCIC 70-194, IntS1 202-392, IntS2 407-641):
+ * order 0 IntCIC, 1 IntS1, 2 IntS2; kind 0 the edge field Ex (at
+ * (i+1/2, j)), 1 the edge field Ey (at (i, j+1/2)), 2 a cell field (at
+ * (i+1/2, j+1/2)); node (0, 0) of the patch sits at `origin`.  s2bug: the
+ * IntS2 gather's (I+1, J+2) read of ScatterGatherImpl.hh:467,525.  Scatter
+ * adds val[k] times the weights into the field. */
+int csrst_sg_gather(int order, int s2bug, int kind, const double *field, int nx, int ny, const double origin[2],
+                    const double dx[2], const double *R, int n, double *out, void *stream);
+int csrst_sg_scatter(int order, int kind, double *field, int nx, int ny, const double origin[2], const double dx[2],
+                     const double *R, const double *val, int n, void *stream);
+/* ShapeFunction<IntOp>::getCurrentDensity{X,Y} with its Base part
+ * (ShapeFunction.hh:30-508; ZerothOrderShapeFunction.cpp:202-294,
+ * FirstOrderShapeFunction.cpp:226-329,540-683,
+ * SecondOrderShapeFunction.cpp:222-345,583-826): the charge-conserving
+ * current of the trajectories oldR -> R over dt, component comp (0 Jx, 1 Jy),
+ * onto the node patch [i0, i0+nx) x [j0, j0+ny) of the mesh whose node
+ * (0, 0) is at `origin`; the patch is cleared first. */
+int csrst_sf_current_density(int order, int comp, double *J, int i0, int j0, int nx, int ny, const double origin[2],
+                             const double dx[2], double dt, const double *oldR, const double *R, const double *Q,
+                             int n, void *stream);
+/* ShapeFunction<IntOp>::getChargeDensity and getChargeDensityDiff
+ * (ZerothOrderShapeFunction.cpp:296-459, FirstOrderShapeFunction.cpp:331-538,
+ * SecondOrderShapeFunction.cpp:347-581): vertex charge density over hx hy on
+ * the node patch (cleared first); Diff: rho(R) - rho(oldR), each particle's
+ * two stencils summed in that order. */
+int csrst_sf_charge_density(int order, double *rho, int i0, int j0, int nx, int ny, const double origin[2],
+                            const double dx[2], const double *R, const double *Q, int n, void *stream);
+int csrst_sf_charge_density_diff(int order, double *rho, int i0, int j0, int nx, int ny, const double origin[2],
+                                 const double dx[2], const double *R, const double *oldR, const double *Q, int n,
+                                 void *stream);
 
 /* ---- Poisson initialisation ------------------------------------------------
  * Replaces PoissonSolver::computeField (PoissonSolver.cpp:85-201), whose

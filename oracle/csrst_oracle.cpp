@@ -1122,6 +1122,44 @@ int poisson_direct(int nx, int ny, double hx, double hy, const uint8_t *flag, co
     return 0;
 }
 
+/* The per-particle parts of PartBunch::push_particles (PartBunch.cpp:215-383):
+ * the half drift R += dt/2 c P / gamma (:228-232) ... */
+inline void half_drift(double *R, const double *P, double half)
+{
+    const V2 Pk = v2(P[0], P[1]);
+    const double g = (double)q_rsqrt((float)(1.0 + dot(Pk, Pk)));
+    R[0] += half * Pk.x * g;
+    R[1] += half * Pk.y * g;
+}
+
+/* ... then the gather at R (:300-358), the kick with fact1 = dt/2 c q/m and
+ * the rotation with B = mu0 Hf + Bext/2 (:360-378), and the second half
+ * drift with the new momentum (:379-383). */
+template <class FX, class FY, class FH>
+void kick_drift(int order, int s2bug, const Bend &geo, double dt, double qm, FX fEX, FY fEY, FH fHZ, V2 oEx, V2 oEy,
+                V2 oHz, V2 dx, double *R, double *P)
+{
+    const double half = 0.5 * dt * C_LIGHT;
+    const double fact1 = half * qm;
+    const double fact2 = fact1 * C_LIGHT;
+    const V2 r = v2(R[0], R[1]);
+    const double ex = gather_edge(order, 0, s2bug, fEX, oEx, dx, r);
+    const double ey = gather_edge(order, 1, s2bug, fEY, oEy, dx, r);
+    const double hf = gather_cell(order, s2bug, fHZ, oHz, dx, r);
+    const double bext = geo.external_field(r.x, r.y);
+    const double B = MU_0 * hf + 0.5 * bext;
+    const V2 kick = v2(fact1 * ex, fact1 * ey);
+    V2 um = v2(P[0] + kick.x, P[1] + kick.y);
+    const double a = fact2 * B * (double)q_rsqrt((float)(1.0 + dot(um, um)));
+    const V2 s = v2(um.x + a * um.y, um.y + a * -um.x);
+    const double tmp = 1.0 + a * a;
+    um.x = ((1.0 + a * a) * s.x + a * s.y) / tmp;
+    um.y = ((1.0 + a * a) * s.y - a * s.x) / tmp;
+    P[0] = um.x + kick.x;
+    P[1] = um.y + kick.y;
+    half_drift(R, P, half);
+}
+
 struct Sim {
     orc_sim_params p;
     Bend *geo = nullptr;
@@ -1530,10 +1568,7 @@ struct Sim {
         for (int k = 0; k < N; ++k) {
             oldR[2 * k] = R[2 * k];
             oldR[2 * k + 1] = R[2 * k + 1];
-            const V2 Pk = v2(P[2 * k], P[2 * k + 1]);
-            const double g = (double)q_rsqrt((float)(1.0 + dot(Pk, Pk)));
-            R[2 * k] += half * Pk.x * g;
-            R[2 * k + 1] += half * Pk.y * g;
+            half_drift(&R[2 * k], &P[2 * k], half);
         }
         /* getFields: patch origins from the bunch domain, PartBunch.cpp:957-1074 */
         int pd[2][2];
@@ -1556,29 +1591,8 @@ struct Sim {
         auto fHZ = [&](long long I, long long Jj) {
             return mH.at(0, fHz[0] + (int)I, fHz[1] + (int)Jj) + mH.at(1, fHz[0] + (int)I, fHz[1] + (int)Jj);
         };
-        const double fact1 = half * qm;
-        const double fact2 = fact1 * C_LIGHT;
-        for (int k = 0; k < N; ++k) {
-            const V2 r = Rp(k);
-            const double ex = gather_edge(p.order, 0, p.s2bug, fEX, oEx, dx, r);
-            const double ey = gather_edge(p.order, 1, p.s2bug, fEY, oEy, dx, r);
-            const double hf = gather_cell(p.order, p.s2bug, fHZ, oHz, dx, r);
-            const double bext = geo->external_field(r.x, r.y);
-            const double B = MU_0 * hf + 0.5 * bext;
-            const V2 kick = v2(fact1 * ex, fact1 * ey);
-            V2 um = v2(P[2 * k] + kick.x, P[2 * k + 1] + kick.y);
-            const double a = fact2 * B * (double)q_rsqrt((float)(1.0 + dot(um, um)));
-            const V2 s = v2(um.x + a * um.y, um.y + a * -um.x);
-            const double tmp = 1.0 + a * a;
-            um.x = ((1.0 + a * a) * s.x + a * s.y) / tmp;
-            um.y = ((1.0 + a * a) * s.y - a * s.x) / tmp;
-            P[2 * k] = um.x + kick.x;
-            P[2 * k + 1] = um.y + kick.y;
-            const V2 Pk = v2(P[2 * k], P[2 * k + 1]);
-            const double g = (double)q_rsqrt((float)(1.0 + dot(Pk, Pk)));
-            R[2 * k] += half * Pk.x * g;
-            R[2 * k + 1] += half * Pk.y * g;
-        }
+        for (int k = 0; k < N; ++k)
+            kick_drift(p.order, p.s2bug, *geo, dt, qm, fEX, fEY, fHZ, oEx, oEy, oHz, dx, &R[2 * k], &P[2 * k]);
         deposit(jcomp);
     }
 
@@ -2324,6 +2338,36 @@ int orc_sim_poisson(void *s, double gamma)
         return -1;
     }
 }
+int orc_push_patch(int order, int s2bug, void *bend, double dt, const double *ex, const double *ey, const double *hz,
+                   int pnx, int pny, const double origin[2], const double dx[2], double *R, double *P, double *oldR,
+                   double *Rmid, int n, int driftBack)
+{
+    try {
+        const Bend &geo = *as<Bend>(bend);
+        const double qm = -1e-9 / M_E_GEV;           /* main.cpp:225 */
+        const double half = 0.5 * dt * C_LIGHT;
+        auto at = [&](const double *f, long long I, long long Jj) {
+            if (I < 0 || Jj < 0 || I >= pnx || Jj >= pny) throw std::out_of_range("gather outside the patch");
+            return f[(size_t)Jj * pnx + (size_t)I];
+        };
+        auto fEX = [&](long long I, long long Jj) { return at(ex, I, Jj); };
+        auto fEY = [&](long long I, long long Jj) { return at(ey, I, Jj); };
+        auto fHZ = [&](long long I, long long Jj) { return at(hz, I, Jj); };
+        const V2 o = v2(origin[0], origin[1]), h = v2(dx[0], dx[1]);
+        for (int k = 0; k < n; ++k) {
+            if (oldR) { oldR[2 * k] = R[2 * k]; oldR[2 * k + 1] = R[2 * k + 1]; }
+            half_drift(&R[2 * k], &P[2 * k], half);
+            kick_drift(order, s2bug, geo, dt, qm, fEX, fEY, fHZ, o, o, o, h, &R[2 * k], &P[2 * k]);
+            if (Rmid) { Rmid[2 * k] = R[2 * k]; Rmid[2 * k + 1] = R[2 * k + 1]; }
+            if (driftBack) half_drift(&R[2 * k], &P[2 * k], -half);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle push: %s\n", e.what());
+        return -1;
+    }
+}
+
 int orc_sim_substep(void *s, int op)
 {
     try {

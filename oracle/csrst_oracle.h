@@ -108,6 +108,17 @@ int   orc_sim_field_steps(void *s, int nsteps);
 /* one part of the step (codes as csrst_substep of include/csrst_hip.h) */
 int   orc_sim_substep(void *s, int op);
 
+/* The particle loop of PartBunch::push_particles (PartBunch.cpp:215-383) on
+ * given fields: half drift, gather of Ex, Ey and Hz = Hzx + Hzy, kick and
+ * rotation with the bend's external field, half drift; then, if driftBack,
+ * the drift by -dt/2 of TETMSolver.hh:210.  The three fields share one patch
+ * [pnx x pny] whose node (0, 0) sits at `origin` (the staggering offsets are
+ * applied by the gathers).  oldR and Rmid (either may be NULL) receive R
+ * before the push and after it (before the drift back: the deposit's end). */
+int   orc_push_patch(int order, int s2bug, void *bend, double dt, const double *ex, const double *ey,
+                     const double *hz, int pnx, int pny, const double origin[2], const double dx[2],
+                     double *R, double *P, double *oldR, double *Rmid, int n, int driftBack);
+
 /* ---- Poisson initialisation (PoissonSolver.cpp:85-201, 678-766) ------ */
 /* Node classes of cutoffStencil on the bend's Nx x Ny grid (row-major);
  * returns maxX = lengthStraightSection in cells. */

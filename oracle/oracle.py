@@ -128,6 +128,26 @@ def deposit(order, comp, box, origin, dx, dt, oldR, R, Q):
     return J
 
 
+def push_patch(order, bend, dt, ex, ey, hz, origin, dx, R, P, s2bug=False, drift_back=True):
+    """The particle loop of PartBunch::push_particles on given fields (one
+    patch [ny, nx] for Ex, Ey and Hz with node (0, 0) at origin), plus the
+    -dt/2 drift after it.  Returns (R, P, oldR, Rmid), Rmid the positions
+    after the push before the drift back (the end of the deposit segment)."""
+    ex, ey, hz = (np.ascontiguousarray(a, np.float64) for a in (ex, ey, hz))
+    R = np.array(R, np.float64).reshape(-1, 2)
+    P = np.array(P, np.float64).reshape(-1, 2)
+    oldR = np.zeros_like(R)
+    Rmid = np.zeros_like(R)
+    ny, nx = ex.shape
+    rc = lib().orc_push_patch(C.c_int(order), C.c_int(int(s2bug)), C.c_void_p(bend.h), C.c_double(dt), _ptr(ex),
+                              _ptr(ey), _ptr(hz), C.c_int(nx), C.c_int(ny), _ptr(np.asarray(origin, np.float64)),
+                              _ptr(np.asarray(dx, np.float64)), _ptr(R), _ptr(P), _ptr(oldR), _ptr(Rmid),
+                              C.c_int(len(R)), C.c_int(int(drift_back)))
+    if rc != 0:
+        raise ValueError("oracle push failed")
+    return R, P, oldR, Rmid
+
+
 BEAM_FIELDS = ("Ekin", "rrms_x", "rrms_y", "prms_x", "prms_y", "emit_x", "emit_y", "rmean_x", "rmean_y",
                "pmean_x", "pmean_y", "Dy", "DDy", "dE", "rmin_x", "rmin_y", "rmax_x", "rmax_y", "rprms_x",
                "rprms_y")
