@@ -17,7 +17,9 @@ SOURCES = ["csrst_ref.hip", "csrst_mw.hip", "csrst_poisson.hip", "csrst_unet.hip
 # The two-grid step and the stateless ShapeFunction / ScatterGatherImpl
 # kernels must stay bit-identical to the reference arithmetic, so their
 # translation units forbid FMA contraction.  The production window, the
-# multigrid and the network are tolerance-checked and use fused multiply-adds.
+# multigrid and the network are tolerance-checked and use fused multiply-adds,
+# except inside `#pragma clang fp contract(off)` scopes (the reference's float
+# fast inverse square root), which -ffp-contract=fast-honor-pragmas keeps.
 CONTRACT = {"csrst_ref.hip": "off", "csrst_sf.hip": "off"}
 ARCH = os.environ.get("CSRST_ARCH", "gfx950")
 
@@ -41,7 +43,7 @@ def build(verbose=True, jobs=4):
     procs = []
     for s in srcs:
         obj = os.path.join(CSRC, s.replace(".hip", ".o"))
-        cmd = [hipcc()] + flags(CONTRACT.get(s, "fast")) + ["-c", os.path.join(CSRC, s), "-o", obj]
+        cmd = [hipcc()] + flags(CONTRACT.get(s, "fast-honor-pragmas")) + ["-c", os.path.join(CSRC, s), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), s))

@@ -115,8 +115,8 @@ def mw_for_c1(fp32):
                             cfl=1.0, pml=0, wakeGrid=True, **GEOM)
 
 
-@pytest.mark.parametrize("fp32,tol", [(False, dict(R=1e-9, P=1e-9, E=1e-8, wake=1e-8)),
-                                      (True, dict(R=2e-3, P=2e-3, E=2e-3, wake=2e-3))])
+@pytest.mark.parametrize("fp32,tol", [(False, dict(R=1e-8, P=1e-8, E=1e-8, wake=1e-10)),
+                                      (True, dict(R=5e-5, P=2e-5, E=1e-4, wake=1e-6))])
 def test_c1_workload_production_vs_oracle(gpu, c1_oracle, fp32, tol):
     """The production window (walls inside it, merged Hz, FMA arithmetic,
     split positions) against the oracle over the same 200 steps.  Errors:
@@ -149,8 +149,20 @@ def test_c1_workload_production_vs_oracle(gpu, c1_oracle, fp32, tol):
     # oracle window cell (i, j) = global cell (pos + i, pos + j) = production logical cell (global - wx0)
     sx, sy = oi["pos"][0] - inf["pos"][0], oi["pos"][1] - inf["pos"][1]
     mE, mH = o["fields"]["mE"], o["fields"]["mH"]
-    sl_o = (slice(3, H - 1), slice(3, W - 1))
-    sl_m = (slice(3 + sy, H - 1 + sy), slice(3 + sx, W - 1 + sx))
+    # profile of the field difference by distance from the oracle window's edges
+    full_o = np.stack([mE[0][1:H + 1, 1:W + 1], mE[1][1:H + 1, 1:W + 1]])
+    full_m = np.stack([F[0][1 + sy:H + 1 + sy, 1 + sx:W + 1 + sx], F[1][1 + sy:H + 1 + sy, 1 + sx:W + 1 + sx]])
+    jj, ii = np.mgrid[0:H, 0:W]
+    dist = np.minimum(np.minimum(ii, W - 1 - ii), np.minimum(jj, H - 1 - jj))
+    dE = np.abs(full_m - full_o).max(0) / np.abs(full_o).max()
+    prof = ["%d-%d: %.1e" % (a, b, dE[(dist >= a) & (dist < b)].max()) for a, b in ((0, 2), (2, 4), (4, 8), (8, 16), (16, 999))]
+    print("  E difference by distance from the oracle window's edge (cells): " + ", ".join(prof))
+    jmax, imax = np.unravel_index(np.argmax(dE), dE.shape)
+    print("  largest at oracle window cell (%d, %d) of %d x %d" % (imax, jmax, W, H))
+    # the comparison keeps 8 cells away from the reference window's edges
+    b8 = 8
+    sl_o = (slice(1 + b8, H + 1 - b8), slice(1 + b8, W + 1 - b8))
+    sl_m = (slice(1 + b8 + sy, H + 1 - b8 + sy), slice(1 + b8 + sx, W + 1 - b8 + sx))
     e_o = np.stack([mE[0][sl_o], mE[1][sl_o]])
     e_m = np.stack([F[0][sl_m], F[1][sl_m]])
     h_o = (mH[0] + mH[1])[sl_o]
