@@ -110,8 +110,11 @@ def test_c1_workload_twogrid_parity(gpu, c1_oracle):
 
 
 def mw_for_c1(fp32):
+    """The production window for the comparison: 512 x 320 cells, tall enough
+    that the chamber's cross-section stays inside it while the bunch turns
+    (a window edge that cuts the chamber reflects like a wall)."""
     b = O.Bend(Nx=C1["Nx"], Ny=C1["Ny"], **GEOM)
-    return MovingWindowStep(512, 256, b.hr[0], b.hr[1], order=C1["order"], fp32=fp32, sortEvery=0, statModulus=1,
+    return MovingWindowStep(512, 320, b.hr[0], b.hr[1], order=C1["order"], fp32=fp32, sortEvery=0, statModulus=1,
                             cfl=1.0, pml=0, wakeGrid=True, **GEOM)
 
 
