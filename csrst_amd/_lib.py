@@ -31,7 +31,8 @@ SYMBOLS = [
     "csrst_mw_download_fields", "csrst_mw_download_particles", "csrst_mw_beam_stats", "csrst_mw_info", "csrst_mw_wake",
     "csrst_mw_timing",
     "csrst_poisson_solve", "csrst_poisson_compute_field", "csrst_mw_poisson",
-    "csrst_sg_gather", "csrst_sg_scatter", "csrst_sf_current_density", "csrst_sf_charge_density",
+    "csrst_sg_gather", "csrst_sg_scatter", "csrst_sf_current_density", "csrst_sf_current_density_noalloc",
+    "csrst_sf_charge_density",
     "csrst_sf_charge_density_diff",
     "csrst_warmstart_create", "csrst_warmstart_destroy", "csrst_warmstart_num_params",
     "csrst_warmstart_get_weights", "csrst_warmstart_set_weights", "csrst_warmstart_infer",

@@ -337,6 +337,37 @@ void check_current_args(int order, int comp, double dt, const double *J, const d
     if (n < 0 || !J || (n > 0 && (!oldR || !R || !Q))) throw Error(CSRST_EINVAL, "bad particle arguments");
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of csrst_sf_current_density_noalloc: one block per device in
+// static device memory, held until the process ends.  In host code `scratch`
+// only names the symbol; its device address comes from hipGetSymbolAddress,
+// once per device with that device current, and is kept per device.
+constexpr size_t kScratchBytes = (size_t)32 << 20;   // 32 MiB
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+unsigned char *device_scratch()
+{
+    int dev = 0;
+    HU_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(CSRST_EINVAL, hu::fmt("device %d beyond the scratch table", dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HU_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    return g_scratch[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -402,6 +433,28 @@ int csrst_sf_current_density(int order, int comp, double *J, int i0, int j0, int
         DBuf<unsigned char> buf;
         buf.alloc(scratch_bytes(cnt, tmp));
         current_density(p, order, comp, J, dt, oldR, R, Q, n, carve(buf.p, cnt, tmp), cnt, (hipStream_t)stream);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return sf_fail(e);
+    }
+}
+
+int csrst_sf_current_density_noalloc(int order, int comp, double *J, int i0, int j0, int nx, int ny,
+                                     const double origin[2], const double dx[2], double dt, const double *oldR,
+                                     const double *R, const double *Q, int n, void *stream)
+{
+    try {
+        check_current_args(order, comp, dt, J, oldR, R, Q, n);
+        const Patch p = make_patch(i0, j0, nx, ny, origin, dx);
+        const size_t cnt = (size_t)n * slots_current(order);
+        const size_t tmp = sort_tmp_bytes(cnt);      // a size query: nothing is allocated
+        const size_t need = scratch_bytes(cnt, tmp);
+        if (need > kScratchBytes)
+            throw Error(CSRST_EINVAL, hu::fmt("%d particles of order %d need %zu bytes of scratch; the no-allocation "
+                                              "entry holds %zu (use csrst_sf_current_density)",
+                                              n, order, need, kScratchBytes));
+        current_density(p, order, comp, J, dt, oldR, R, Q, n, carve(device_scratch(), cnt, tmp), cnt,
+                        (hipStream_t)stream);
         return CSRST_OK;
     } catch (const std::exception &e) {
         return sf_fail(e);

@@ -109,16 +109,17 @@ def scatter(order, kind, shape, origin, dx, R, val, field=None):
     return f.get()
 
 
-def getCurrentDensity(order, comp, box, origin, dx, dt, oldR, R, Q):
+def getCurrentDensity(order, comp, box, origin, dx, dt, oldR, R, Q, noalloc=False):
     """ShapeFunction<IntOp>::getCurrentDensity{X,Y} onto the node patch
-    box = (i0, j0, nx, ny) of the mesh with node 0 at origin; returns J[ny, nx]."""
+    box = (i0, j0, nx, ny) of the mesh with node 0 at origin; returns J[ny, nx].
+    noalloc: through csrst_sf_current_density_noalloc (static scratch)."""
     i0, j0, nx, ny = box
     O, Rn = _pairs(oldR), _pairs(R)
     J = DeviceArray(shape=(ny, nx))
     o, r, q = DeviceArray(O), DeviceArray(Rn), DeviceArray(np.asarray(Q, np.float64))
-    check(_lib.lib().csrst_sf_current_density(C.c_int(order), C.c_int(comp), J.ptr, C.c_int(i0), C.c_int(j0),
-                                              C.c_int(nx), C.c_int(ny), _v2(origin), _v2(dx), C.c_double(dt), o.ptr,
-                                              r.ptr, q.ptr, C.c_int(len(Rn)), None))
+    fn = _lib.lib().csrst_sf_current_density_noalloc if noalloc else _lib.lib().csrst_sf_current_density
+    check(fn(C.c_int(order), C.c_int(comp), J.ptr, C.c_int(i0), C.c_int(j0), C.c_int(nx), C.c_int(ny), _v2(origin),
+             _v2(dx), C.c_double(dt), o.ptr, r.ptr, q.ptr, C.c_int(len(Rn)), None))
     return J.get()
 
 

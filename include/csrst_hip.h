@@ -240,6 +240,18 @@ int csrst_sg_scatter(int order, int kind, double *field, int nx, int ny, const d
 int csrst_sf_current_density(int order, int comp, double *J, int i0, int j0, int nx, int ny, const double origin[2],
                              const double dx[2], double dt, const double *oldR, const double *R, const double *Q,
                              int n, void *stream);
+/* csrst_sf_current_density without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (sort keys and values, the sort's temporary storage, an error word)
+ * is a static 32 MiB block of device memory per GPU, held until the process
+ * ends; inputs needing more (about 27k particles of S2, 55k of S1, 160k of
+ * S0) return CSRST_EINVAL.  Calls that overlap on one GPU share that block
+ * and race: issue them one after the other.  Results equal
+ * csrst_sf_current_density's bit for bit. */
+int csrst_sf_current_density_noalloc(int order, int comp, double *J, int i0, int j0, int nx, int ny,
+                                     const double origin[2], const double dx[2], double dt, const double *oldR,
+                                     const double *R, const double *Q, int n, void *stream);
 /* ShapeFunction<IntOp>::getChargeDensity and getChargeDensityDiff
  * (ZerothOrderShapeFunction.cpp:296-459, FirstOrderShapeFunction.cpp:331-538,
  * SecondOrderShapeFunction.cpp:347-581): vertex charge density over hx hy on
