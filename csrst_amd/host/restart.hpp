@@ -8,6 +8,8 @@
 #include <sys/wait.h>
 #include <unistd.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -157,6 +159,13 @@ inline Bunch restore_fields(Engine &e, const ckpt::Record &r)
 {
     const Commands &c = e.commands();
     if (r.Nx != c.numgridpx || r.Ny != c.numgridpy) throw Error(CSRST_EINVAL, "restart file grid size");
+    // restoreFields takes the file's mesh spacing and origin (LoadH5Part.cpp:223-233);
+    // the engine's mesh comes from the bend arguments, which the command line may
+    // have changed: the stored fields only fit the mesh they were written on
+    auto near = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)); };
+    if (!near(r.hx, e.hr()[0]) || !near(r.hy, e.hr()[1]) || !near(r.ox, e.origin()[0]) || !near(r.oy, e.origin()[1]))
+        throw Error(CSRST_EINVAL, "restart file mesh (spacing and origin) differs from the mesh of the bend "
+                                  "arguments; give the arguments the file was written with");
     const size_t nx = (size_t)r.Nx, ny = (size_t)r.Ny, pitch = nx + 2, plane = pitch * (ny + 2);
     const std::vector<double> *src[2] = {&r.E, &r.H};
     for (int f = 0; f < 2; ++f) {

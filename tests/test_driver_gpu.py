@@ -219,3 +219,8 @@ def test_driver_wake_files(gpu, tmp_path):
         got = open(tmp_path / "Data" / name).read()
         assert got == text, name
     assert not os.path.exists(tmp_path / "Data" / "energyChangeInBend.dat")
+    # a width on the command line overrides the file's: the bend's mesh then
+    # differs from the one the stored fields were written on, which is refused
+    r = subprocess.run([EXE, "-r", "run.h5part", "--width", "0.0132"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "mesh" in (r.stdout + r.stderr)

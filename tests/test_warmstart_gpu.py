@@ -14,9 +14,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
-def test_forward_and_gradients_match_cpu(gpu):
+@pytest.mark.parametrize("H,W,B", [(16, 24, 2), (32, 48, 2)])
+def test_forward_and_gradients_match_cpu(gpu, H, W, B):
+    """At 32 x 48 x 2 the top level holds 3072 pixels, more than one
+    1024-pixel chunk of the weight-gradient kernel: its partial sums from
+    several chunks are added."""
     rng = np.random.default_rng(1)
-    H, W, B = 16, 24, 2
     u = WarmStart(H, W, max_batch=B)
     p = u.weights()
     p[np.concatenate([np.arange(l["boff"], l["boff"] + l["cout"]) for l in layers()[0]])] = \
