@@ -25,6 +25,7 @@ SYMBOLS = [
     "csrst_download_field", "csrst_upload_field", "csrst_download_particles", "csrst_download_charges",
     "csrst_info",
     "csrst_wake", "csrst_wake_reset", "csrst_is_inside_bend", "csrst_diag", "csrst_beam_stats", "csrst_field_energy",
+    "csrst_line_fields", "csrst_gauss_residual",
     "csrst_mw_create", "csrst_mw_destroy", "csrst_mw_init_bunch", "csrst_mw_step", "csrst_mw_push", "csrst_mw_set_stream",
     "csrst_mw_place",
     "csrst_mw_upload_fields",

@@ -670,6 +670,21 @@ __global__ void k_scatter_q(Parts p, double o0, double o1, double h0, double h1,
         }
 }
 
+// Gauss-law residual on the window's vertices: div E - rho / eps0 with the
+// divergence of the edge fields as PartBunch::calcChargeConservation forms
+// the divergence of J (PartBunch.cpp:552-556: value at i minus value at i-1
+// over the spacing) and rho the bunch's vertex charge density (getChargeDensity,
+// in `rho` over the cell area).
+__global__ void k_gauss(FieldDesc E, const double *rho, double ihx, double ihy, double area, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= E.n) return;
+    const double div = (E.c0[fidx(E, i, j)] - E.c0[fidx(E, i - 1, j)]) * ihx +
+                       (E.c1[fidx(E, i, j)] - E.c1[fidx(E, i, j - 1)]) * ihy;
+    const size_t q = (size_t)j * E.n + i;
+    out[q] = div - rho[q] / area / pm::kEps0;
+}
+
 // fillRHS (PoissonSolver.cpp:441-460): b = rho / (hx hy) / (eps0 gamma).
 __global__ void k_rhs(const double *rho, double *b, size_t n, double hh, double cc)
 {
@@ -748,6 +763,43 @@ __global__ void k_reset_outside(FieldDesc E, const uint8_t *code)
     if (c & 1) E.c0[fidx(E, i, j)] = 0.0;
     if (c & 2) E.c1[fidx(E, i, j)] = 0.0;
 }
+
+namespace {
+// SaveEMFieldsCommand::execute (SaveEMFields.cpp:99-133): for the grid
+// columns i0 + t, the point of the line through R along P, the gathers of E,
+// H (both split parts) and J on the global fields with the run's shape, and
+// the projections along and across P.  Values outside the global fields'
+// guard layer read as zero.
+__global__ void k_line(FieldDesc E, FieldDesc H, FieldDesc J, int order, int s2bug, double o0, double o1, double h0,
+                       double h1, double R0, double R1, double P0, double P1, int i0, int n, double *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = i0 + t;
+    const double x = o0 + i * h0;
+    const double y = R1 + (o0 + i * h0 - R0) / P0 * P1;
+    const vec2<double> od = pm::mk(1.0 / h0, 1.0 / h1);
+    const vec2<double> pp = pm::mk((x - o0) * od.x, (y - o1) * od.y);
+    auto at = [](const FieldDesc &f, const double *c) {
+        return [&f, c](long long I, long long Jj) {
+            if (I < -1 || I > f.n || Jj < -1 || Jj > f.m) return 0.0;
+            return c[fidx(f, (int)I, (int)Jj)];
+        };
+    };
+    const bool bug = s2bug != 0;
+    const pm::Stencil<double> sx = pm::edge_stencil(order, 0, pp), sy = pm::edge_stencil(order, 1, pp);
+    const pm::Stencil<double> sc = pm::cell_stencil(order, pp);
+    const double ex = pm::gather(sx, bug, at(E, E.c0)), ey = pm::gather(sy, bug, at(E, E.c1));
+    const double ha = pm::gather(sc, bug, at(H, H.c0)), hb = pm::gather(sc, bug, at(H, H.c1));
+    const double jx = pm::gather(sx, bug, at(J, J.c0)), jy = pm::gather(sy, bug, at(J, J.c1));
+    const double lenP = sqrt(P0 * P0 + P1 * P1);
+    const double cosP = P0 / lenP, sinP = P1 / lenP;
+    out[4 * t] = cosP * ex + sinP * ey;
+    out[4 * t + 1] = -sinP * ex + cosP * ey;
+    out[4 * t + 2] = ha + hb;
+    out[4 * t + 3] = cosP * jx + sinP * jy;
+}
+}  // namespace
 
 struct csrst_ctx {
     std::string err;
@@ -1888,6 +1940,109 @@ int csrst_beam_stats(csrst_ctx *c, double *out)
         bs::raw_stats(BeamSrc{c->parts.rx, c->parts.ry, c->parts.px, c->parts.py}, c->n, c->s, c->bsPart.p,
                       c->bsLim.p, raw);
         bs::parameters(raw, out);
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_line_fields(csrst_ctx *c, double *out, int cap, int *count)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "not initialised");
+        double st[CSRST_BEAM_STATS];
+        const int rc = csrst_beam_stats(c, st);
+        if (rc != CSRST_OK) return rc;
+        const double R0 = st[7], R1 = st[8], P0 = st[9], P1 = st[10];
+        const double h0 = c->hr[0], h1 = c->hr[1], o0 = c->bend.origin[0], o1 = c->bend.origin[1];
+        const int Nx = c->p.Nx, Ny = c->p.Ny;
+        // the line's column range from the bunch's cells (SaveEMFields.cpp:55-71)
+        const int gb0 = (int)floor((st[14] - o0) / h0), gb1 = (int)floor((st[16] - o0) / h0);
+        double l0 = floor(1.5 * gb0 - 0.5 * gb1), u0 = 2 * gb1 - gb0;
+        double l1 = (R1 + (o0 + l0 * h0 - R0) / P0 * P1 - o1) / h1;
+        double u1 = (R1 + (o0 + u0 * h0 - R0) / P0 * P1 - o1) / h1;
+        const long startIdx = (long)l0;
+        const int ns = (int)floor(u0 - l0 + 0.5) + 1;
+        if (count) *count = ns;
+        if (!out) return CSRST_OK;
+        if (cap < ns) throw Error(CSRST_EINVAL, "capacity too small");
+        // getPartInside (SaveEMFields.cpp:162-189) on the one-rank field domain
+        {
+            const int lower = (int)floor(l0 + 0.5), upper = (int)floor(u0 + 0.5);
+            int minL = upper, maxL = upper;
+            const double p0 = u0 - l0, p1 = u1 - l1;
+            for (int i = lower; i <= upper; ++i) {
+                const int yi = (int)floor(l1 + (i - lower) / p0 * p1);
+                if (i >= 0 && i < Nx && yi >= 0 && yi < Ny) {
+                    if (i < minL) minL = i;
+                    if (i > minL) maxL = i;   // as written at SaveEMFields.cpp:180
+                }
+            }
+            const double t1 = l1 + (minL - lower) / p0 * p1;
+            u1 = l1 + (maxL - lower) / p0 * p1;
+            u0 = maxL;
+            l0 = minL;
+            l1 = t1;
+        }
+        int dom[2][2] = {{(int)floor(l0 + 0.5), (int)floor(u0 + 0.5)}, {(int)floor(l1), (int)floor(u1)}};
+        const int fdom[2][2] = {{0, Nx - 1}, {0, Ny - 1}};
+        bool inside = true;
+        for (int d = 0; d < 2; ++d) {
+            const int lo = std::max(fdom[d][0], dom[d][0]), hi = std::min(fdom[d][1], dom[d][1]);
+            if (lo > hi) { inside = false; break; }
+            dom[d][0] = lo; dom[d][1] = hi;
+        }
+        std::vector<double> line(4 * (size_t)ns, 0.0);
+        if (inside) {
+            const int n = dom[0][1] - dom[0][0] + 1;
+            DBuf<double> dev;
+            dev.alloc(4 * (size_t)n);
+            k_line<<<blocks(n), kBlock, 0, c->s>>>(c->fs.f[F_GE], c->fs.f[F_GH], c->fs.f[F_GJ], c->p.order,
+                                                   c->p.s2bug, o0, o1, h0, h1, R0, R1, P0, P1, dom[0][0], n, dev.p);
+            HU_LAUNCH_CHECK();
+            HU_CHECK(hipMemcpyAsync(line.data() + 4 * (size_t)(dom[0][0] - startIdx), dev.p, 4 * (size_t)n * 8,
+                                    hipMemcpyDeviceToHost, c->s));
+            HU_CHECK(hipStreamSynchronize(c->s));
+        }
+        long ii = startIdx;
+        for (int i = 0; i < ns; ++i, ++ii) {
+            const double c0 = o0 + ii * h0 - R0, c1 = c0 / P0 * P1;
+            double *row = out + 7 * (size_t)i;
+            row[0] = sqrt(c0 * c0 + c1 * c1) * (c0 > 0 ? 1 : -1);
+            for (int k = 0; k < 4; ++k) row[1 + k] = line[4 * (size_t)i + k];
+            row[5] = c0 + R0;
+            row[6] = c1 + R1;
+        }
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int csrst_gauss_residual(csrst_ctx *c, double *out, int cap, int *count)
+{
+    try {
+        if (!c || !c->ready) throw Error(CSRST_ESTATE, "not initialised");
+        const int W = c->W, H = c->H;
+        const size_t nn = (size_t)W * H;
+        if (count) *count = (int)nn;
+        if (!out) return CSRST_OK;
+        if ((size_t)cap < nn) throw Error(CSRST_EINVAL, "capacity too small");
+        DBuf<double> rho, res;
+        rho.alloc(nn);
+        res.alloc(nn);
+        rho.zero(c->s);
+        const double o0 = c->mmo[0], o1 = c->mmo[1], h0 = c->hr[0], h1 = c->hr[1];
+        if (c->p.order == 0)
+            k_scatter_q<0><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, h0, h1, rho.p, W, H);
+        else if (c->p.order == 1)
+            k_scatter_q<1><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, h0, h1, rho.p, W, H);
+        else
+            k_scatter_q<2><<<blocks(c->n), kBlock, 0, c->s>>>(c->parts, o0, o1, h0, h1, rho.p, W, H);
+        k_gauss<<<grid2(W, H), kBlock, 0, c->s>>>(c->fs.f[F_ME], rho.p, 1.0 / h0, 1.0 / h1, h0 * h1, res.p);
+        HU_LAUNCH_CHECK();
+        HU_CHECK(hipMemcpyAsync(out, res.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HU_CHECK(hipStreamSynchronize(c->s));
         return CSRST_OK;
     } catch (const std::exception &e) {
         return fail(c, e);

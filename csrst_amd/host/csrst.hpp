@@ -16,6 +16,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -423,6 +424,36 @@ private:
 
     CalcEnergy cec_;
     std::ofstream out_;
+};
+
+// SaveEMFieldsCommand (SaveEMFields.cpp:34-160): Data/FieldOnLine_NNNN.dat,
+// one file per call, numbered from 0; per line s, E along and across the
+// mean momentum, Hz, J along it, x, y, tab-separated in the stream's default
+// six digits (csrst_line_fields computes them on the GPU).
+class SaveEMFields {
+public:
+    explicit SaveEMFields(const Engine &e) : e_(e) {}
+    void execute()
+    {
+        int n = 0;
+        check(csrst_line_fields(e_.ctx(), nullptr, 0, &n), e_.ctx());
+        std::vector<double> rows(7 * (size_t)(n > 0 ? n : 1));
+        check(csrst_line_fields(e_.ctx(), rows.data(), n, &n), e_.ctx());
+        char name[64];
+        std::snprintf(name, sizeof(name), "Data/FieldOnLine_%04u.dat", iteration_);
+        std::ofstream out(name);
+        if (!out) throw Error(CSRST_EINVAL, std::string("cannot write ") + name);
+        for (int i = 0; i < n; ++i) {
+            const double *r = &rows[7 * (size_t)i];
+            out << r[0] << "\t" << r[1] << "\t" << r[2] << "\t" << r[3] << "\t" << r[4] << "\t" << r[5] << "\t"
+                << r[6] << "\n";
+        }
+        ++iteration_;
+    }
+
+private:
+    const Engine &e_;
+    unsigned iteration_ = 0;
 };
 
 class TETMSolver {

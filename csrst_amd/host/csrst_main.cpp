@@ -129,8 +129,7 @@ int main(int argc, char **argv)
             plot(engine, pavc, mmpavc, 0.0);
             if (saving) saver->write(csrst::restart::snapshot(engine, bunch, t, cmd.startstep));
         }
-        if (cmd.linemodulus > 0 && cmd.linemodulus <= cmd.numsteps)
-            std::printf("csrst: --linemodulus: the field copy is done, the SaveEMFields line files are not written\n");
+        csrst::SaveEMFields semfc(engine);
         bool inStraightSectionBeforeBend = true, inBend = false;   // main.cpp:91-92
         const auto t0 = std::chrono::steady_clock::now();
         for (int it = cmd.startstep; it < cmd.startstep + cmd.numsteps; ++it) {
@@ -169,9 +168,12 @@ int main(int argc, char **argv)
                     inBend = false;
                 }
             }
-            // main.cpp:468-476: the line dump copies E and H first
-            if (cmd.linemodulus > 0 && it % cmd.linemodulus == cmd.linemodulus - 1 && !plotStep)
-                engine.copyFields(false);
+            // main.cpp:468-476: the line dump (SaveEMFieldsCommand) after the window's
+            // E and H are copied into the global fields
+            if (cmd.linemodulus > 0 && it % cmd.linemodulus == cmd.linemodulus - 1) {
+                if (!plotStep) engine.copyFields(false);
+                semfc.execute();
+            }
             engine.moveMesh();   // wake sample, J = 0, moveMesh (main.cpp:445-501)
             t += engine.dt();
         }

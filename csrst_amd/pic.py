@@ -225,6 +225,26 @@ class TwoGridStep:
         check(_lib.lib().csrst_diag(self.h, dptr(out)), self.h)
         return out
 
+    def lineFields(self):
+        """SaveEMFieldsCommand::execute (SaveEMFields.cpp:34-160) on the global
+        fields: rows [n, 7] = s, E along P, E across P, Hz, J along P, x, y."""
+        L = _lib.lib()
+        cnt = C.c_int(0)
+        check(L.csrst_line_fields(self.h, None, 0, C.byref(cnt)), self.h)
+        out = np.zeros(7 * max(cnt.value, 1))
+        check(L.csrst_line_fields(self.h, dptr(out), cnt.value, C.byref(cnt)), self.h)
+        return out[:7 * cnt.value].reshape(cnt.value, 7)
+
+    def gaussResidual(self):
+        """div E - rho/eps0 on the window's vertices, [H, W] (csrst_gauss_residual)."""
+        L = _lib.lib()
+        cnt = C.c_int(0)
+        check(L.csrst_gauss_residual(self.h, None, 0, C.byref(cnt)), self.h)
+        out = np.zeros(max(cnt.value, 1))
+        check(L.csrst_gauss_residual(self.h, dptr(out), cnt.value, C.byref(cnt)), self.h)
+        inf = self.info()
+        return out[:cnt.value].reshape(inf["H"], inf["W"])
+
     def fieldEnergySums(self):
         """(sum E.E, sum (Hzx+Hzy)^2) x cell area over global + window - overlap,
         the sums of CalcEnergyCommand::execute (CalcEnergy.cpp:60-124)."""

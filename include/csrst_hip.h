@@ -144,6 +144,24 @@ int csrst_diag(csrst_ctx *c, double *out);
  * into the frame of the mean momentum; double-double sums. */
 #define CSRST_BEAM_STATS 20
 int csrst_beam_stats(csrst_ctx *c, double *out);
+/* SaveEMFieldsCommand::execute (SaveEMFields.cpp:34-160): the fields on the
+ * line through the bunch centroid along its mean momentum, at the global
+ * grid's columns from 1.5 min - 0.5 max to 2 max - min of the bunch's cell
+ * range, gathered from the global E, H, J with the run's shape
+ * (ScatterGatherImpl semantics; the reference's IPPL gather is not
+ * available to pin them).  Rows of 7 doubles: s (signed distance from the
+ * centroid), E along P, E across P, Hzx + Hzy, J along P, x, y -- the
+ * columns of Data/FieldOnLine_NNNN.dat.  The reference copies the window
+ * into the global fields first (csrst_copy_fields(c, 0), main.cpp:468-476).
+ * out == NULL: *count receives the row count. */
+int csrst_line_fields(csrst_ctx *c, double *out, int cap, int *count);
+/* Gauss's law on the window: per window vertex (i, j), row-major [H][W],
+ * (Ex(i,j) - Ex(i-1,j))/hx + (Ey(i,j) - Ey(i,j-1))/hy - rho(i,j)/eps0, with
+ * rho the bunch's getChargeDensity of the run's order at R (the divergence is
+ * formed as PartBunch::calcChargeConservation forms div J, PartBunch.cpp:540-582).
+ * Its change from step to step is the charge-conservation drift of the field
+ * solve.  out == NULL: *count receives W*H. */
+int csrst_gauss_residual(csrst_ctx *c, double *out, int cap, int *count);
 /* The field sums of CalcEnergyCommand::execute (CalcEnergy.cpp:60-124) over the
  * global grid plus the window minus the global nodes under the window, times
  * the cell area: out[0] = sum E.E, out[1] = sum (Hzx + Hzy)^2.  The command's

@@ -2338,6 +2338,97 @@ int orc_sim_poisson(void *s, double gamma)
         return -1;
     }
 }
+/* SaveEMFieldsCommand::execute (SaveEMFields.cpp:34-160) on the global
+ * fields, with getPartInside (:162-189) on a one-rank field domain.  The
+ * state is PartBunch::calcBeamParameters (beam_parameters above): rmean,
+ * pmean, rmin, rmax.  IPPL's ParticleAttrib::gather with IntOp_t is taken as
+ * ScatterGatherImpl<IntOp_t> (edge semantics for E and J, cell semantics for
+ * H, each split component gathered and the two H parts summed as :131).
+ * Rows of 7: s, E along P, E across P, Hz, J along P, x, y (the file's
+ * columns, :141-149).  Returns the number of rows (-1: cap too small). */
+int orc_sim_line_fields(void *sp, double *out, int cap)
+{
+    Sim *s = as<Sim>(sp);
+    double st[20];
+    beam_parameters(s->R.data(), s->P.data(), s->N, st);
+    const V2 R = v2(st[7], st[8]), P = v2(st[9], st[10]);
+    const V2 h = v2(s->hr[0], s->hr[1]);
+    const V2 origin = v2(s->geo->origin[0], s->geo->origin[1]);
+    /* getCellContaining of rmin and rmax */
+    const int gb0 = (int)std::floor((st[14] - origin.x) / h.x), gb1 = (int)std::floor((st[16] - origin.x) / h.x);
+    V2 l, u;
+    l.x = std::floor(1.5 * gb0 - 0.5 * gb1);
+    u.x = 2 * gb1 - gb0;
+    l.y = (R.y + (origin.x + l.x * h.x - R.x) / P.x * P.y - origin.y) / h.y;
+    u.y = (R.y + (origin.x + u.x * h.x - R.x) / P.x * P.y - origin.y) / h.y;
+    const long startIdx = (long)l.x;
+    const unsigned numSamples = (unsigned)std::floor(u.x - l.x + 0.5) + 1;
+    if ((int)numSamples > cap) return -1;
+    std::vector<double> line(4 * (size_t)numSamples, 0.0);
+    /* getPartInside: the first and last column whose line point lies in the field domain */
+    V2 lc = l, uc = u;
+    {
+        const int lower = (int)std::floor(lc.x + 0.5), upper = (int)std::floor(uc.x + 0.5);
+        int minL = upper, maxL = upper;
+        const V2 proj = v2(uc.x - lc.x, uc.y - lc.y);
+        for (int i = lower; i <= upper; ++i) {
+            const double y = lc.y + (i - lower) / proj.x * proj.y;
+            const int yi = (int)std::floor(y);
+            if (i >= 0 && i < s->p.Nx && yi >= 0 && yi < s->p.Ny) {
+                if (i < minL) minL = i;
+                if (i > minL) maxL = i;        /* as written (:180) */
+            }
+        }
+        const V2 tmp = v2(minL, lc.y + (minL - lower) / proj.x * proj.y);
+        uc = v2(maxL, lc.y + (maxL - lower) / proj.x * proj.y);
+        lc = tmp;
+    }
+    int dom[2][2] = {{(int)std::floor(lc.x + 0.5), (int)std::floor(uc.x + 0.5)},
+                     {(int)std::floor(lc.y), (int)std::floor(uc.y)}};
+    const int fdom[2][2] = {{0, s->p.Nx - 1}, {0, s->p.Ny - 1}};
+    bool inside = true;
+    for (int d = 0; d < 2; ++d) {
+        const int lower = std::max(fdom[d][0], dom[d][0]), upper = std::min(fdom[d][1], dom[d][1]);
+        if (lower > upper) { inside = false; break; }
+        dom[d][0] = lower; dom[d][1] = upper;
+    }
+    if (inside) {
+        auto fetch = [&](const VField &f, int k) {
+            return [&f, k](long long I, long long J) {
+                return f.in((int)I, (int)J) ? f.at(k, (int)I, (int)J) : 0.0;
+            };
+        };
+        const double lenP = std::sqrt(dot(P, P));
+        const double cosP = P.x / lenP, sinP = P.y / lenP;
+        for (int i = dom[0][0]; i <= dom[0][1]; ++i) {
+            const V2 pos = v2(origin.x + i * h.x, R.y + (origin.x + i * h.x - R.x) / P.x * P.y);
+            const int o = s->p.order, bug = s->p.s2bug;
+            const double ex = gather_edge(o, 0, bug, fetch(s->E, 0), origin, h, pos);
+            const double ey = gather_edge(o, 1, bug, fetch(s->E, 1), origin, h, pos);
+            const double h0 = gather_cell(o, bug, fetch(s->H, 0), origin, h, pos);
+            const double h1 = gather_cell(o, bug, fetch(s->H, 1), origin, h, pos);
+            const double jx = gather_edge(o, 0, bug, fetch(s->J, 0), origin, h, pos);
+            const double jy = gather_edge(o, 1, bug, fetch(s->J, 1), origin, h, pos);
+            const size_t ii = (size_t)(i - startIdx);
+            line[4 * ii] = cosP * ex + sinP * ey;
+            line[4 * ii + 1] = -sinP * ex + cosP * ey;
+            line[4 * ii + 2] = h0 + h1;
+            line[4 * ii + 3] = cosP * jx + sinP * jy;
+        }
+    }
+    long ii = startIdx;
+    for (unsigned i = 0; i < numSamples; ++i, ++ii) {
+        const double c0 = origin.x + ii * h.x - R.x;
+        const double c1 = c0 / P.x * P.y;
+        double *row = out + 7 * (size_t)i;
+        row[0] = std::sqrt(c0 * c0 + c1 * c1) * (c0 > 0 ? 1 : -1);
+        row[1] = line[4 * i]; row[2] = line[4 * i + 1]; row[3] = line[4 * i + 2]; row[4] = line[4 * i + 3];
+        row[5] = c0 + R.x;
+        row[6] = c1 + R.y;
+    }
+    return (int)numSamples;
+}
+
 int orc_push_patch(int order, int s2bug, void *bend, double dt, const double *ex, const double *ey, const double *hz,
                    int pnx, int pny, const double origin[2], const double dx[2], double *R, double *P, double *oldR,
                    double *Rmid, int n, int driftBack)

@@ -119,6 +119,11 @@ int   orc_push_patch(int order, int s2bug, void *bend, double dt, const double *
                      const double *hz, int pnx, int pny, const double origin[2], const double dx[2],
                      double *R, double *P, double *oldR, double *Rmid, int n, int driftBack);
 
+/* SaveEMFieldsCommand::execute (SaveEMFields.cpp:34-160) on the Sim's
+ * global E, H, J: rows of 7 (s, E along P, E across P, Hz, J along P, x, y);
+ * returns the row count, -1 if cap is too small. */
+int   orc_sim_line_fields(void *s, double *out, int cap);
+
 /* ---- Poisson initialisation (PoissonSolver.cpp:85-201, 678-766) ------ */
 /* Node classes of cutoffStencil on the bend's Nx x Ny grid (row-major);
  * returns maxX = lengthStraightSection in cells. */

@@ -370,6 +370,15 @@ class Sim:
         for g, w in (("E", "mE"), ("H", "mH")) + ((("J", "mJ"),) if copyCurrent else ()):
             self.set_field(g, copy_window(self.field(g), self.field(w), pos))
 
+    def line_fields(self, cap=100000):
+        """SaveEMFieldsCommand::execute (SaveEMFields.cpp:34-160) on the global
+        fields: rows [n, 7] = s, E along P, E across P, Hz, J along P, x, y."""
+        out = np.zeros(7 * cap)
+        n = lib().orc_sim_line_fields(C.c_void_p(self.h), _ptr(out), C.c_int(cap))
+        if n < 0:
+            raise ValueError("line longer than cap")
+        return out[:7 * n].reshape(n, 7)
+
     def field_energy_sums(self):
         """CalcEnergyCommand sums (CalcEnergy.cpp:60-124): sum E.E and sum
         (Hzx+Hzy)^2 over global + window - overlap, times the cell area."""

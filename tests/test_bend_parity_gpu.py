@@ -239,3 +239,69 @@ def test_production_push_vs_oracle(gpu, fp32, comp, tilt):
         assert rows > 100
     tol = (2e-5, 2e-6, 2e-5) if fp32 else (1e-11, 1e-12, 1e-12)
     assert err_p < tol[0] and err_r < tol[1] and err_j < tol[2]
+
+
+def test_line_fields_vs_oracle(gpu):
+    """SaveEMFieldsCommand (SaveEMFields.cpp:34-160) through csrst_line_fields
+    after 40 steps into the bend and the copy of the window's E and H that
+    precedes it (main.cpp:468-476), against the oracle.  The gathers follow
+    ScatterGatherImpl; the reference's own IPPL gather is parity unpinned."""
+    Nx, Ny = 1100, 120
+    R, P, Q = gaussian_bunch(2000, (0.046, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=6)
+    g, o = twogrid_pair(2, Nx, Ny, R, P, Q, stat=1)
+    for _ in range(40):
+        g.step(1)
+        o.step(1)
+    g.copyFields(False)
+    o.copy_fields(False)
+    lg, lo = g.lineFields(), o.line_fields()
+    assert lg.shape == lo.shape and len(lg) > 50
+    for col in range(7):
+        scale = np.abs(lo[:, col]).max()
+        if scale > 0:
+            assert np.abs(lg[:, col] - lo[:, col]).max() <= 1e-12 * scale, col
+    assert np.abs(lo[:, 1]).max() > 0 and np.abs(lo[:, 3]).max() > 0
+
+
+def test_gauss_residual(gpu):
+    """csrst_gauss_residual: div E - rho/eps0 on the window's vertices.  At
+    the -I start (E = 0) it is -rho/eps0 with the oracle's charge density;
+    after steps into the bend it equals the same formula on the downloaded
+    fields and particles.  Prints how far Gauss's law drifts from the start
+    (the split step updates Ex and Ey with H of different half steps, and
+    the pushes deposit Jx and Jy half a step apart, so the reference's scheme
+    does not hold it at round-off)."""
+    eps0 = 8.854187817e-12
+    Nx, Ny = 1100, 120
+    b = O.Bend(Nx=Nx, Ny=Ny, **GEOM)
+    hx, hy = b.hr
+    R, P, Q = gaussian_bunch(3000, (0.046, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=8)
+    g, _ = twogrid_pair(1, Nx, Ny, R, P, Q, stat=1)
+
+    def expected():
+        inf = g.info()
+        Rg, _, _ = g.particles()
+        rho = O.charge_density(1, (0, 0, inf["W"], inf["H"]), inf["origin"], (hx, hy), Rg, g.charges())
+        E = g.field("mE")
+        div = (E[0][1:-1, 1:-1] - E[0][1:-1, :-2]) / hx + (E[1][1:-1, 1:-1] - E[1][:-2, 1:-1]) / hy
+        return div - rho / eps0, np.abs(rho).max() / eps0
+
+    G0 = g.gaussResidual()
+    want, scale = expected()
+    assert np.abs(G0 - want).max() <= 1e-12 * scale
+    pos0 = g.info()["pos"]
+    drift = []
+    for k in range(40):
+        g.step(1)
+        if k % 10 == 9:
+            G = g.gaussResidual()
+            want, scale = expected()
+            assert np.abs(G - want).max() <= 1e-11 * scale
+            (px, py), H, W = g.info()["pos"], *G.shape
+            sx, sy = px - pos0[0], py - pos0[1]
+            a = G[max(0, -sy):H - max(0, sy), max(0, -sx):W - max(0, sx)]
+            b0 = G0[max(0, sy):H - max(0, -sy), max(0, sx):W - max(0, -sx)]
+            n0, n1 = min(a.shape[0], b0.shape[0]), min(a.shape[1], b0.shape[1])
+            drift.append(np.abs(a[:n0, :n1] - b0[:n0, :n1]).max() / scale)
+    print("Gauss residual drift from the start, relative to max rho/eps0, after 10/20/30/40 steps: %s"
+          % ", ".join("%.2e" % d for d in drift), flush=True)

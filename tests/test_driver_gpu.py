@@ -172,6 +172,11 @@ def test_driver_save_and_restart(gpu, tmp_path):
     for k, v in (("x", Rg[:, 0]), ("y", Rg[:, 1]), ("oldx", Og[:, 0]), ("oldy", Og[:, 1]), ("px", Pg[:, 0]),
                  ("py", Pg[:, 1]), ("q", g.charges())):
         assert np.array_equal(rec12[k], v), k
+    # a width on the command line overrides the file's: the bend's mesh then
+    # differs from the one the stored fields were written on, which is refused
+    r = subprocess.run([EXE, "-r", "run.h5part", "--width", "0.0132"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "mesh" in (r.stdout + r.stderr)
 
 
 def test_driver_wake_files(gpu, tmp_path):
@@ -219,8 +224,31 @@ def test_driver_wake_files(gpu, tmp_path):
         got = open(tmp_path / "Data" / name).read()
         assert got == text, name
     assert not os.path.exists(tmp_path / "Data" / "energyChangeInBend.dat")
-    # a width on the command line overrides the file's: the bend's mesh then
-    # differs from the one the stored fields were written on, which is refused
-    r = subprocess.run([EXE, "-r", "run.h5part", "--width", "0.0132"], cwd=str(tmp_path), capture_output=True,
-                       text=True, timeout=120)
-    assert r.returncode != 0 and "mesh" in (r.stdout + r.stderr)
+
+
+def test_driver_line_files(gpu, tmp_path):
+    """--linemodulus 4: Data/FieldOnLine_0000.dat and _0001.dat after steps 4
+    and 8 (main.cpp:468-476), each line the library's csrst_line_fields row
+    of the same run printed as the reference's stream does (six digits)."""
+    if not os.path.exists(EXE):
+        pytest.skip("driver not built")
+    R, P, Q = gaussian_bunch(2000, (0.02, 0.0064), 1e-3, 0.5e-3, 100.0, 1.0, seed=19, energy_spread=1e-3)
+    run_driver(tmp_path, R, P, Q, ["--numsteps", "9", "--linemodulus", "4"])
+    cmd = Commands(numgridpx=1100, numgridpy=120, lengthbefore=GEOM["lengthbefore"], length=GEOM["length"],
+                   width=GEOM["width"], phi=GEOM["phi"], Ekin=GEOM["Ekin"], Bz=GEOM["Bz"], order=1)
+    g = TwoGridStep(cmd)
+    g.init_bunch(R, P, Q)
+    files = []
+    for it in range(9):
+        g.updateField_firstStep()
+        g.updateField_secondStep()
+        if it % 4 == 3:
+            g.copyFields(False)
+            rows = g.lineFields()
+            files.append("".join("\t".join("%g" % v for v in r) + "\n" for r in rows))
+        g.moveMesh()
+    data = tmp_path / "Data"
+    assert sorted(n for n in os.listdir(data) if n.startswith("FieldOnLine")) == ["FieldOnLine_0000.dat",
+                                                                                 "FieldOnLine_0001.dat"]
+    for k, want in enumerate(files):
+        assert (data / ("FieldOnLine_%04d.dat" % k)).read_text() == want

@@ -257,3 +257,44 @@ def test_split_field_step_moves_div_e():
     change = np.abs(div(E1) - d0)[inner].max() / (np.abs(E1[0]).max() / hx)
     print("div E change in one field step: %.2e of max|E|/hx" % change)
     assert change > 1e-5
+
+
+def test_line_fields_projection_and_geometry():
+    """SaveEMFieldsCommand (SaveEMFields.cpp:34-160) on uniform fields: every
+    sample is the projection of the field onto the mean momentum and its
+    normal (the gathers reproduce constants), the samples sit on the global
+    grid's columns on the line through the centroid along the mean momentum,
+    s is the signed distance along it, and the columns run from
+    floor(1.5 gmin - 0.5 gmax) to 2 gmax - gmin of the bunch's cell range."""
+    geo = dict(lenBefore=0.05, lenAfter=0.01, width=0.0128, phiDeg=10.0, ekinMeV=50.59, bz=0.1704)
+    s = O.Sim(1100, 120, order=1, statModulus=1, **geo)
+    th = np.radians(5.0)
+    rng = np.random.default_rng(3)
+    n = 500
+    R = np.stack([0.03 + rng.normal(0, 1e-3, n), 0.006 + rng.normal(0, 0.5e-3, n)], 1)
+    P = np.stack([np.full(n, 99.9 * np.cos(th)), np.full(n, 99.9 * np.sin(th))], 1) + rng.normal(0, 0.05, (n, 2))
+    s.init(R, P, np.full(n, -1e-12))
+    shape = (2, 122, 1102)
+    for name, vals in (("E", (3.0, -2.0)), ("H", (0.25, 0.5)), ("J", (7.0, 1.0))):
+        f = np.zeros(shape)
+        f[0], f[1] = vals
+        s.set_field(name, f)
+    rows = s.line_fields()
+    Rs, Ps, _ = s.particles()
+    Rm, Pm = Rs.mean(0), Ps.mean(0)
+    c, sn = Pm / np.hypot(*Pm)
+    b = O.Bend(Nx=1100, Ny=120, **geo)
+    assert np.allclose(rows[:, 1], c * 3.0 + sn * -2.0, rtol=1e-14, atol=0)
+    assert np.allclose(rows[:, 2], -sn * 3.0 + c * -2.0, rtol=1e-14, atol=0)
+    assert np.allclose(rows[:, 3], 0.75, rtol=1e-14, atol=0)
+    assert np.allclose(rows[:, 4], c * 7.0 + sn * 1.0, rtol=1e-14, atol=0)
+    x, y = rows[:, 5], rows[:, 6]
+    dx, dy = x - Rm[0], y - Rm[1]
+    assert np.abs(dx * Pm[1] - dy * Pm[0]).max() < 1e-12 * np.hypot(*Pm) * np.abs(dx).max()
+    assert np.allclose(rows[:, 0], np.sign(dx) * np.hypot(dx, dy), rtol=1e-12, atol=1e-15)
+    cols = (x - b.origin[0]) / b.hr[0]
+    assert np.abs(cols - np.round(cols)).max() < 1e-9
+    gmin = np.floor((Rs[:, 0].min() - b.origin[0]) / b.hr[0])
+    gmax = np.floor((Rs[:, 0].max() - b.origin[0]) / b.hr[0])
+    assert round(cols[0]) == np.floor(1.5 * gmin - 0.5 * gmax) and round(cols[-1]) == 2 * gmax - gmin
+    assert np.all(np.diff(np.round(cols)) == 1)
