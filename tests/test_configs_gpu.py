@@ -1,0 +1,120 @@
+"""The benchmark's workloads run at their full sizes on the GPU.
+
+c1, c2, c3, c5 (the production moving window, FP32 split precision and
+FP64 on the same seeded bunch): the runs finish without a window error, the
+windows move together and follow the bunch round the bend, the FP32 CSR
+wake stays within 1e-4 of the FP64 wake (c5: 1e-2, see CASES) and the FP32
+centroid and mean momentum within 1e-3 cells and 1e-6 of FP64's.  c2 and c3 start half-way round the arc.
+c1 (CFL 0.5, 16 PML columns, first-order shape) and c5 start with the
+Poisson field of the bunch, which the reference computes for a bunch moving
+along the first straight section (PoissonSolver.cpp:64-71 boosts along x):
+they start before the bend and run into it.
+
+c4: the warm-start network's inference on 2048 x 1024 densities and the
+multigrid from its guess to the 1e-10 residual on held-out densities.
+"""
+import numpy as np
+import pytest
+
+from csrst_amd.configs import bunch_for, bunch_in_bend, config
+from csrst_amd.mw import MovingWindowStep
+
+pytestmark = pytest.mark.gpu
+
+C_LIGHT = 299792458.0
+# (workload, steps, where the bunch starts: a fraction of the arc, or metres
+# before the bend entry, FP32 wake tolerance against FP64)
+# c5: the Poisson start makes the nodes past the straight section identity
+# rows holding rho / (eps0 gamma) (PoissonSolver.cpp:441-460 over
+# cutoffStencil's rows), so the bunch starts 2 mm (6.7 sigma_s) before the
+# bend.  Its FP32 wake is measured at 5.9e-3 of FP64's: from the Poisson
+# start the bunch carries its full transverse field, about gamma^2 = 9e4
+# times the longitudinal field the wake samples, and the FP32 rounding of Hz
+# (6e-8 of it) enters Ex through the curl each step: 6e-8 x 9e4 = 5e-3.
+CASES = [("c1", 2000, -0.01, 1e-4), ("c2", 1000, 0.5, 1e-4), ("c3", 1000, 0.5, 1e-4), ("c5", 3000, -2e-3, 1e-2)]
+
+
+def run(cfg, fp32, R, P, Q, steps, stat):
+    m = MovingWindowStep(cfg["W"], cfg["H"], cfg["dx"], cfg["dy"], cfg["lenBefore"], cfg["lenAfter"], cfg["width"],
+                         cfg["phiDeg"], cfg["ekinMeV"], cfg["bz"], order=cfg["order"], fp32=fp32,
+                         sortEvery=cfg["sortEvery"], statModulus=stat, cfl=cfg["cfl"], pml=cfg["pml"])
+    m.init_bunch(R, P, Q)
+    pois = m.poisson(tol=1e-10, maxit=100) if cfg.get("poisson") else None
+    pos0 = m.info()["pos"]
+    s0 = m.calcBeamParameters()
+    m.step(steps)
+    out = dict(wake=m.wake(), info=m.info(), pos0=pos0, s0=s0, s1=m.calcBeamParameters(), poisson=pois)
+    m.close()
+    return out
+
+
+@pytest.mark.parametrize("name,steps,where,wtol", CASES, ids=[c[0] for c in CASES])
+def test_workload_in_bend(gpu, name, steps, where, wtol):
+    cfg = config(name)
+    print("%s: %d particles, %d x %d cells, %d steps from %s" % (
+        name, cfg["n"], cfg["W"], cfg["H"], steps,
+        "%.0f %% of the arc" % (100 * where) if where > 0 else "%g mm before the bend" % (-1e3 * where)), flush=True)
+    if where > 0:
+        R, P, Q = bunch_in_bend(cfg, seed=3, arc_fraction=where)
+    else:
+        R, P, Q = bunch_for(cfg, seed=3, s_offset=where)
+    r64 = run(cfg, False, R, P, Q, steps, 20)
+    r32 = run(cfg, True, R, P, Q, steps, 20)
+    del R, P, Q
+    # the windows moved together and followed the bunch
+    assert r64["info"]["pos"] == r32["info"]["pos"]
+    dt = r64["info"]["dt"]
+    th = np.arctan2(r64["s0"]["pmean_y"], r64["s0"]["pmean_x"])
+    moved = np.array(r64["info"]["pos"]) - np.array(r64["pos0"])
+    travel = steps * C_LIGHT * dt
+    assert abs(moved[0] * cfg["dx"] - travel * np.cos(th)) < 0.05 * travel + 3 * cfg["dx"]
+    assert abs(moved[1] * cfg["dy"] - travel * np.sin(th)) < 0.05 * travel + 3 * cfg["dy"]
+    # the bunch turned with the bend: the mean momentum rotated by the arc
+    # length travelled inside it over R
+    th1 = np.arctan2(r64["s1"]["pmean_y"], r64["s1"]["pmean_x"])
+    inside = travel if where > 0 else travel + where
+    assert abs((th1 - th) - inside / cfg["R"]) < 0.1 * inside / cfg["R"] + 1e-6
+    # FP32 against FP64
+    w64, w32 = r64["wake"], r32["wake"]
+    werr = np.abs(w32 - w64).max() / np.abs(w64).max()
+    cerr = max(abs(r32["s1"][k] - r64["s1"][k]) for k in ("rmean_x", "rmean_y")) / cfg["dx"]
+    perr = max(abs(r32["s1"][k] - r64["s1"][k]) for k in ("pmean_x", "pmean_y")) / np.hypot(
+        r64["s1"]["pmean_x"], r64["s1"]["pmean_y"])
+    print("%s: FP32 vs FP64 after %d steps: wake %.2e, centroid %.2e cells, mean momentum %.2e; window moved %s%s"
+          % (name, steps, werr, cerr, perr, tuple(moved),
+             "; Poisson V-cycles %s" % r64["poisson"]["cycles"] if r64["poisson"] else ""), flush=True)
+    assert werr < wtol
+    assert cerr < 1e-3 and perr < 1e-6
+
+
+def test_c4_inference_and_multigrid(gpu):
+    """The network's guess for 2048 x 1024 densities at full size, then the
+    multigrid from that guess to the 1e-10 residual (the fallback and judge:
+    it converges from any guess)."""
+    from csrst_amd import poisson as PS
+    from csrst_amd.warmstart import WarmStart
+    H, W, h = 1024, 2048, 10e-6
+    rng = np.random.default_rng(404)
+    y, x = np.mgrid[0:H, 0:W]
+    cls = np.zeros((H, W), np.uint8)
+    cls[1:-1, 1:-1] = PS.UNKNOWN
+    u = WarmStart(H, W, max_batch=2)
+    for seed in range(2):
+        rho = np.zeros((H, W))
+        k = rng.integers(1, 5)
+        amps = rng.random(k)
+        amps /= amps.sum()
+        for a in amps:
+            s = rng.uniform(20e-6, 500e-6) / h
+            cx, cy = rng.uniform(0.3 * W, 0.7 * W), rng.uniform(0.3 * H, 0.7 * H)
+            rho += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s)) / (2 * np.pi * s * s)
+        rho[cls != PS.UNKNOWN] = 0.0
+        guess = u.infer(rho[None].astype(np.float32), scale=1.0 / np.abs(rho).max())[0]
+        assert guess.shape == (H, W) and np.isfinite(guess).all()
+        phi, cyc, res = PS.solve(h, h, cls, rho, phi0=guess.astype(np.float64), tol=1e-10, maxit=200)
+        phi0, cyc0, res0 = PS.solve(h, h, cls, rho, tol=1e-10, maxit=200)
+        print("c4 seed %d: V-cycles from the guess %d, from zero %d, residual %.1e" % (seed, cyc, cyc0, res),
+              flush=True)
+        assert res <= 1e-10 and res0 <= 1e-10
+        assert np.abs(phi - phi0).max() <= 1e-8 * np.abs(phi0).max()
+    u.close()
