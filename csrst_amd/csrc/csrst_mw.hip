@@ -522,9 +522,11 @@ __device__ __forceinline__ void rotate_in(T (&o)[kRun], T v)
 // not pay: the kernel has to fit the instruction cache.
 // At most three waves per SIMD: at four (the register count allows it for
 // FP32 S2) the push runs 10 % slower, at two 27 % slower.
-// MISSCOOP: slot misses through the wavefront (a tilted bunch: its misses are
-// many and share anchors) instead of per-lane LDS atomics (an axial bunch:
-// misses are rare, and the cooperative path in the loop costs 20 %).
+// MISSCOOP: slot misses through the wavefront instead of per-lane LDS
+// atomics.  Which form is faster depends on the bunch's angle and density
+// (c3 at 1-10 deg: the cooperative form 13-25 % faster; c2 at 1-8 deg: 2-8 %
+// slower; along x the cooperative path in the loop costs 20 %), so the step
+// times the two against each other and keeps the faster (csrst_mw_step).
 template <class T, int ORDER, int COMP, bool MISSCOOP>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3))) k_push(Parts<T> p, Fields<T> f, Geo g, PushConst<T> c, WinState *ws)
 {
@@ -663,12 +665,18 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
             int gx = gx0, gy = gy0;
             T fx = fx0, fy = fy0;
             T g2 = 0;
+            T x1 = fx0, y1 = fy0, x2 = fx0, y2 = fy0;
             if (live) {
                 P = pm::mk(p0.x + dP.x, p0.y + dP.y);
                 // half drift (PartBunch.cpp:228-232)
                 const T gi = pm::inv_gamma(P);
-                split_add(gx, fx, c.hx * P.x * gi);
-                split_add(gy, fy, c.hy * P.y * gi);
+                x1 = fx0 + c.hx * P.x * gi;
+                y1 = fy0 + c.hy * P.y * gi;
+                {
+                    const T flx = pm::floor_(x1), fly = pm::floor_(y1);
+                    gx = gx0 + (int)flx; gy = gy0 + (int)fly;
+                    fx = x1 - flx; fy = y1 - fly;
+                }
                 // gather at R1 (positions relative to the particle's cell corner).
                 // With fx, fy in [0, 1) every stencil of the three fields starts at
                 // node offset L or L + 1 from the cell and spans N nodes, so the
@@ -739,12 +747,12 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
                 dP = kick_rotate_split(p0, dP, ex, ey, hf, bext, c.fact1, c.fact2);
                 P = pm::mk(p0.x + dP.x, p0.y + dP.y);
                 g2 = pm::inv_gamma(P);
-                split_add(gx, fx, c.hx * P.x * g2);
-                split_add(gy, fy, c.hy * P.y * g2);
+                x2 = x1 + c.hx * P.x * g2;
+                y2 = y1 + c.hy * P.y * g2;
             }
             // ---- deposit of R0 -> R2, cell units relative to the old cell corner
             const vec2<T> o = pm::mk(fx0, fy0);
-            const vec2<T> e = pm::mk((T)(gx - gx0) + fx, (T)(gy - gy0) + fy);
+            const vec2<T> e = pm::mk(x2, y2);
             Split<T> xs;
             const bool oneCross = split_path(xs, o, e, ORDER == 1);   // dead lanes: o = e = 0
             if (live && !oneCross) err |= 2;
@@ -809,10 +817,12 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
             }
             // ---- drift back by half a step (TETMSolver.hh:210, 265)
             if (live) {
-                bminx = min(bminx, min(gx0, gx)); bminy = min(bminy, min(gy0, gy));
-                bmaxx = max(bmaxx, max(gx0, gx)); bmaxy = max(bmaxy, max(gy0, gy));
-                split_add(gx, fx, -c.hx * P.x * g2);
-                split_add(gy, fy, -c.hy * P.y * g2);
+                const int ex2 = gx0 + (int)pm::floor_(x2), ey2 = gy0 + (int)pm::floor_(y2);
+                bminx = min(bminx, min(gx0, ex2)); bminy = min(bminy, min(gy0, ey2));
+                bmaxx = max(bmaxx, max(gx0, ex2)); bmaxy = max(bmaxy, max(gy0, ey2));
+                gx = gx0; gy = gy0; fx = T(0); fy = T(0);
+                split_add(gx, fx, x2 - c.hx * P.x * g2);
+                split_add(gy, fy, y2 - c.hy * P.y * g2);
             }
             rotate_in(GX, gx); rotate_in(GY, gy);
             rotate_in(FX, fx); rotate_in(FY, fy);
@@ -1511,10 +1521,12 @@ struct csrst_mw {
 
     int cur = 0;   // which of the Ex/Hz (and PML psi) ping-pong buffers is current
     int npml = 0;  // PML columns at each window end
-    // the bunch's reference momentum points more than kTiltDeg off the x axis
-    // (set at set-up and after each batch of steps from the device's value):
-    // the push sends slot misses through the wavefront
+    // the push sends slot misses through the wavefront (MISSCOOP).  Chosen by
+    // timing: every kRetune steps one step runs each form and the faster stays
     bool tilted = false;
+    int tunePhase = 0;            // 0 idle, 1 timing the cooperative form, 2 the per-lane form
+    int nextTune = 0;             // step at which the next comparison starts
+    float tuneMs = 0;             // push time of the cooperative form
     DBuf<unsigned char> pmlBuf;   // psih, psih', psie planes + [4][W] coefficients
 
     template <class T>
@@ -1697,20 +1709,16 @@ void one_step(csrst_mw *m, bool timed)
     ++m->steps;
 }
 
-// measured crossover of the two push forms at c2 (FP32): equal near 15 deg;
-// at 30 deg the cooperative form is 20 % faster, along x 22 % slower
-constexpr double kTiltDeg = 15.0;
-bool tilted(const double p0[2])
-{
-    return std::fabs(p0[1]) > std::tan(kTiltDeg * M_PI / 180.0) * std::fabs(p0[0]);
-}
+// the two push forms are compared again after this many steps (the bunch's
+// angle and density change slowly; one comparison costs at most one step in
+// the slower form)
+constexpr int kRetune = 200;
 
 void check_state(csrst_mw *m)
 {
     mw::WinState h;
     HU_CHECK(hipMemcpyAsync(&h, m->ws.p, sizeof(h), hipMemcpyDeviceToHost, m->s));
     HU_CHECK(hipStreamSynchronize(m->s));
-    m->tilted = tilted(h.p0);
     if (h.err & 1) throw Error(CSRST_EWINDOW, "particle stencil left the window");
     if (h.err & 2) throw Error(CSRST_EWINDOW, "particle crossed too many cell lines in one push");
 }
@@ -1947,7 +1955,9 @@ int csrst_mw_init_bunch(csrst_mw *m, const double *R, const double *P, const dou
         HU_CHECK(hipMemcpy(m->partBuf.p, host.data(), host.size(), hipMemcpyHostToDevice));
         mw::k_state_init<<<1, 1, 0, m->s>>>(m->ws.p, wx0, wy0);
         mw::k_set_p0<<<1, 1, 0, m->s>>>(m->ws.p, p0[0], p0[1]);
-        m->tilted = tilted(p0);
+        m->tilted = false;
+        m->tunePhase = 0;
+        m->nextTune = 0;
         const dim3 gg((m->W + 2 * mw::G + mw::kB - 1) / mw::kB, m->H + 2 * mw::G);
         if (f32) hipLaunchKernelGGL((mw::k_geom_all<float>), gg, dim3(mw::kB), 0, m->s, m->fields<float>(), m->geo, m->ws.p);
         else hipLaunchKernelGGL((mw::k_geom_all<double>), gg, dim3(mw::kB), 0, m->s, m->fields<double>(), m->geo, m->ws.p);
@@ -2008,6 +2018,10 @@ int csrst_mw_step(csrst_mw *m, int nsteps)
         int nsorted = 0;
         for (int k = 0; k < nsteps; ++k) {
             const bool sorted = m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0;
+            if (m->n > 0 && m->tunePhase == 0 && m->steps >= m->nextTune) {
+                m->tunePhase = 1;
+                m->tilted = true;
+            }
             if (m->f32) one_step<float>(m, true); else one_step<double>(m, true);
             HU_CHECK(hipEventSynchronize(m->ev[5]));
             float t01, t12, t23, t34, t45, t67 = 0;
@@ -2016,6 +2030,15 @@ int csrst_mw_step(csrst_mw *m, int nsteps)
             HU_CHECK(hipEventElapsedTime(&t23, m->ev[2], m->ev[3]));
             HU_CHECK(hipEventElapsedTime(&t34, m->ev[3], m->ev[4]));
             HU_CHECK(hipEventElapsedTime(&t45, m->ev[4], m->ev[5]));
+            if (m->tunePhase == 1) {          // this step ran the cooperative form
+                m->tuneMs = t01 + t23;
+                m->tunePhase = 2;
+                m->tilted = false;
+            } else if (m->tunePhase == 2) {   // and this one the per-lane form
+                m->tilted = m->tuneMs < t01 + t23;
+                m->tunePhase = 0;
+                m->nextTune = m->steps + kRetune;
+            }
             if (sorted) { HU_CHECK(hipEventElapsedTime(&t67, m->ev[6], m->ev[7])); ++nsorted; }
             acc[0] += t01 + t23;
             acc[1] += t12 + t34;
