@@ -201,20 +201,19 @@ def copy_probe(mib):
 
 
 def measured_traffic(workload):
-    """HBM bytes per k_push launch from the committed rocprofv3 --pmc summary,
-    when it was measured on this workload; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "k_push_c2_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    kv = {}
-    with open(path) as f:
-        for line in f:
-            if ":" in line:
-                k, v = line.split(":", 1)
-                kv[k.strip()] = v.strip()
-    if kv.get("workload") != workload or "hbm_bytes_per_launch" not in kv:
-        return None
-    return float(kv["hbm_bytes_per_launch"])
+    """HBM bytes per k_push launch from the committed rocprofv3 --pmc summary
+    (profiles/k_push_*_traffic.txt) measured on this workload; None if none was."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "k_push_*_traffic.txt"))):
+        kv = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("workload") == workload and "hbm_bytes_per_launch" in kv:
+            return float(kv["hbm_bytes_per_launch"])
+    return None
 
 
 def main():
