@@ -15,6 +15,7 @@
 // workgroups with FP32 atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <string>
@@ -55,6 +56,30 @@ __device__ __forceinline__ sh4 load4(const bf16 *p)
     r[0] = (short)(v.x & 0xFFFF); r[1] = (short)(v.x >> 16);
     r[2] = (short)(v.y & 0xFFFF); r[3] = (short)(v.y >> 16);
     return r;
+}
+
+// eight bf16 values as floats (one 16-byte load or store)
+struct V8 { float v[8]; };
+
+__device__ __forceinline__ V8 ld8(const bf16 *p)
+{
+    const uint4 u = *reinterpret_cast<const uint4 *>(p);
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    V8 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r.v[2 * i] = __uint_as_float(w[i] << 16);
+        r.v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void st8(bf16 *p, const V8 &r)
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = (uint32_t)f2bf(r.v[2 * i]) | ((uint32_t)f2bf(r.v[2 * i + 1]) << 16);
+    *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 struct ConvArgs {
@@ -174,10 +199,309 @@ __global__ void __launch_bounds__(kB) k_conv(ConvArgs a)
         }
 }
 
-// k_conv with the widest channel blocking that divides CoutP (small Cin
-// takes the flattened path, one tile)
+// ---------------------------------------------------------------------------
+// The 3x3 convolution for Cin a multiple of 32 and Cout of 16 MF on the
+// gfx950 v_mfma_f32_16x16x32_bf16 (K = 32 input channels of one tap per MFMA),
+// persistent: a workgroup of 8 waves keeps one block of 16 MF output channels
+// and walks its work items (a 16 x 32 pixel tile, a chunk of 32 input
+// channels) tile by tile.  Per item the (16+2) x (32+2) input pixels with their
+// halo and the 16 MF x 9 x 32 weights sit in LDS and each wave runs 9 taps x
+// MF x 4 MFMAs on tile rows 2w and 2w+1.  The global loads of the next DEPTH
+// items are in flight in registers while the current one computes (DEPTH 2
+// where one item's matrix work is too short to cover the memory latency);
+// with Cin = 32 the weights are staged once for all tiles.  An LDS pixel (or
+// weight row) holds 32 channels in 64 B padded to 80 B, so the 16-byte reads
+// of eight consecutive lanes fall on distinct banks.  Lane l holds
+// A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16] (one 16-byte LDS read each)
+// and D[4(l/16)..+3][l%16].
+constexpr int kTW = 32;                       // output tile: pixels per row
+constexpr int kIW = kTW + 2;                  // input tile row with the halo
+constexpr int kPS = 40;                       // LDS stride of 32 channels, in bf16
+constexpr int kPH = 16;                       // output rows per tile
+constexpr int kPIH = kPH + 2;                 // input rows with the halo
+constexpr int kPB = 512;                      // threads: 8 waves
+constexpr int kXSeg = kPIH * kIW * 4;         // 16-byte segments of one input tile chunk
+constexpr int kXPer = (kXSeg + kPB - 1) / kPB;
+constexpr int kTH = 8, kIH = kTH + 2;         // 8 x 32 pixel tiles of the one-channel layers
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+inline int conv32p_tiles(const ConvArgs &a)
+{
+    return a.B * ((a.H + kPH - 1) / kPH) * ((a.W + kTW - 1) / kTW);
+}
+
+template <int MF, int DEPTH>
+__global__ void __launch_bounds__(kPB, 1) k_conv32p(ConvArgs a)
+{
+    constexpr int kWSeg = 9 * 16 * MF * 4;    // 16-byte segments of one weight chunk
+    constexpr int kWPer = (kWSeg + kPB - 1) / kPB;
+    __shared__ __attribute__((aligned(16))) bf16 sX[kPIH * kIW * kPS];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * 16 * MF * kPS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kg = lane >> 4;
+    const int nCo = a.CoutP / (16 * MF);
+    const int tilesX = (a.W + kTW - 1) / kTW, tilesY = (a.H + kPH - 1) / kPH;
+    const int nTiles = a.B * tilesX * tilesY;
+    const int cb = blockIdx.x % nCo, g = blockIdx.x / nCo, G = gridDim.x / nCo;
+    const int co0 = cb * 16 * MF;
+    const int Cin = a.Cin, nChunks = Cin / 32;
+    if (g >= nTiles) return;
+    const int nItems = ((nTiles - g + G - 1) / G) * nChunks;   // tiles g, g + G, ... times chunks
+    const bool wOnce = nChunks == 1;
+    uint4 rxA[kXPer], rwA[kWPer], rxB[DEPTH == 2 ? kXPer : 1], rwB[DEPTH == 2 ? kWPer : 1];
+    f4 acc[MF][4];
+#pragma unroll
+    for (int m = 0; m < MF; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+    // this lane's biases in registers: a global load in the epilogue would
+    // make its wait also wait for the prefetch in flight
+    float bias[MF][4];
+#pragma unroll
+    for (int m = 0; m < MF; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bias[m][r] = a.bias ? a.bias[co0 + 16 * m + 4 * kg + r] : 0.f;
+    // item -> (image, tile origin, first input channel)
+#define CONV32P_ORIGIN(item)                                       \
+    const int tile_ = g + ((item) / nChunks) * G;                  \
+    const int c0 = ((item) % nChunks) * 32;                        \
+    const int x0 = (tile_ % tilesX) * kTW;                         \
+    const int y0 = ((tile_ / tilesX) % tilesY) * kPH;              \
+    const int bimg = (tile_ / tilesX) / tilesY;
+    // global loads of an item into rx (and rw)
+#define CONV32P_FETCH(item, withW, rx, rw)                                                                           \
+    {                                                                                                                \
+        CONV32P_ORIGIN(item)                                                                                         \
+        _Pragma("unroll") for (int j = 0; j < kXPer; ++j)                                                            \
+        {                                                                                                            \
+            const int s = tid + kPB * j;                                                                             \
+            const int pix = s >> 2, seg = s & 3;                                                                     \
+            const int iy = pix / kIW, ix = pix - iy * kIW;                                                           \
+            const int ys = y0 + iy - 1, xs = x0 + ix - 1;                                                            \
+            const bool in = s < kXSeg && ys >= 0 && ys < a.H && xs >= 0 && xs < a.W;                                 \
+            rx[j] = in ? *reinterpret_cast<const uint4 *>(a.x + ((size_t)(bimg * a.H + ys) * a.W + xs) * Cin + c0 +  \
+                                                          8 * seg)                                                   \
+                       : make_uint4(0u, 0u, 0u, 0u);                                                                 \
+        }                                                                                                            \
+        if (withW) {                                                                                                 \
+            _Pragma("unroll") for (int j = 0; j < kWPer; ++j)                                                        \
+            {                                                                                                        \
+                const int s = tid + kPB * j;                                                                         \
+                const int seg = s & 3, r = s >> 2;                                                                   \
+                const int col = r / 9, tap = r - 9 * col;                                                            \
+                rw[j] = s < kWSeg ? *reinterpret_cast<const uint4 *>(a.w + ((size_t)(co0 + col) * 9 + tap) * Cin +  \
+                                                                     c0 + 8 * seg)                                   \
+                                  : make_uint4(0u, 0u, 0u, 0u);                                                      \
+            }                                                                                                        \
+        }                                                                                                            \
+    }
+    // one item: its registers into LDS, the loads of item + DEPTH issued, the
+    // MFMAs, and after the tile's last chunk bias, ReLU, bf16 store
+#define CONV32P_STEP(item, rx, rw)                                                                                   \
+    {                                                                                                                \
+        const bool loadW = !wOnce || (item) == 0;                                                                    \
+        __syncthreads();                                                                                             \
+        _Pragma("unroll") for (int j = 0; j < kXPer; ++j)                                                            \
+        {                                                                                                            \
+            const int s = tid + kPB * j;                                                                             \
+            if (s < kXSeg) *reinterpret_cast<uint4 *>(&sX[(s >> 2) * kPS + 8 * (s & 3)]) = rx[j];                    \
+        }                                                                                                            \
+        if (loadW)                                                                                                   \
+            _Pragma("unroll") for (int j = 0; j < kWPer; ++j)                                                        \
+            {                                                                                                        \
+                const int s = tid + kPB * j;                                                                         \
+                const int seg = s & 3, r = s >> 2;                                                                   \
+                const int col = r / 9, tap = r - 9 * col;                                                            \
+                if (s < kWSeg) *reinterpret_cast<uint4 *>(&sW[(tap * 16 * MF + col) * kPS + 8 * seg]) = rw[j];       \
+            }                                                                                                        \
+        __syncthreads();                                                                                             \
+        if ((item) + DEPTH < nItems) CONV32P_FETCH((item) + DEPTH, !wOnce, rx, rw)                                   \
+        _Pragma("unroll") for (int tap = 0; tap < 9; ++tap)                                                          \
+        {                                                                                                            \
+            const int dy = a.flip ? 1 - tap / 3 : tap / 3 - 1, dx = a.flip ? 1 - tap % 3 : tap % 3 - 1;              \
+            bf8 av[MF], bv[4];                                                                                       \
+            _Pragma("unroll") for (int m = 0; m < MF; ++m) av[m] =                                                   \
+                *reinterpret_cast<const bf8 *>(&sW[(tap * 16 * MF + 16 * m + li) * kPS + 8 * kg]);                   \
+            _Pragma("unroll") for (int t = 0; t < 4; ++t)                                                            \
+            {                                                                                                        \
+                const int ry = 2 * wave + (t >> 1) + 1 + dy, rxx = 16 * (t & 1) + li + 1 + dx;                      \
+                bv[t] = *reinterpret_cast<const bf8 *>(&sX[(ry * kIW + rxx) * kPS + 8 * kg]);                        \
+            }                                                                                                        \
+            _Pragma("unroll") for (int m = 0; m < MF; ++m)                                                           \
+                _Pragma("unroll") for (int t = 0; t < 4; ++t) acc[m][t] = mfma32(av[m], bv[t], acc[m][t]);           \
+        }                                                                                                            \
+        if ((item) % nChunks == nChunks - 1) {                                                                       \
+            CONV32P_ORIGIN(item)                                                                                     \
+            (void)c0;                                                                                                \
+            _Pragma("unroll") for (int t = 0; t < 4; ++t)                                                            \
+            {                                                                                                        \
+                const int yy = y0 + 2 * wave + (t >> 1), xx = x0 + 16 * (t & 1) + li;                               \
+                const bool ok = yy < a.H && xx < a.W;                                                                \
+                const size_t p = ((size_t)bimg * a.H + yy) * a.W + xx;                                               \
+                _Pragma("unroll") for (int m = 0; m < MF; ++m)                                                       \
+                {                                                                                                    \
+                    const int co = co0 + 16 * m + 4 * kg;                                                            \
+                    float v[4];                                                                                      \
+                    _Pragma("unroll") for (int r = 0; r < 4; ++r)                                                    \
+                    {                                                                                                \
+                        v[r] = acc[m][t][r] + bias[m][r];                                                            \
+                        if (a.relu) v[r] = v[r] > 0.f ? v[r] : 0.f;                                                  \
+                    }                                                                                                \
+                    acc[m][t] = f4{0.f, 0.f, 0.f, 0.f};                                                              \
+                    if (ok && a.y) {                                                                                 \
+                        uint2 o;                                                                                     \
+                        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);                                   \
+                        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);                                   \
+                        *reinterpret_cast<uint2 *>(a.y + p * a.Cout + co) = o;                                       \
+                    }                                                                                                \
+                    if (ok && a.yf)                                                                                  \
+                        _Pragma("unroll") for (int r = 0; r < 4; ++r) a.yf[p * a.Cout + co + r] = v[r];              \
+                }                                                                                                    \
+            }                                                                                                        \
+        }                                                                                                            \
+    }
+    CONV32P_FETCH(0, true, rxA, rwA)
+    if (DEPTH == 2 && nItems > 1) CONV32P_FETCH(1, !wOnce, rxB, rwB)
+    for (int item = 0; item < nItems; item += DEPTH) {
+        CONV32P_STEP(item, rxA, rwA)
+        if (DEPTH == 2 && item + 1 < nItems) CONV32P_STEP(item + 1, rxB, rwB)
+    }
+#undef CONV32P_STEP
+#undef CONV32P_FETCH
+#undef CONV32P_ORIGIN
+}
+
+// 3x3 convolution from one input channel to Cout (a multiple of 8, at most
+// 256) channels: one thread per pixel; the weights as floats in LDS, [tap][co],
+// read by every lane at one address.  Load-bound: 2 B in and 2 Cout B out per
+// pixel.
+__global__ void __launch_bounds__(kB) k_conv_c1(ConvArgs a)
+{
+    __shared__ float sw[9 * 256];
+    for (int k = threadIdx.x; k < a.Cout * 9; k += kB) {
+        const int tap = k / a.Cout, co = k - tap * a.Cout;
+        sw[k] = bf2f(a.w[co * 9 + tap]);
+    }
+    __syncthreads();
+    const int p = blockIdx.x * kB + threadIdx.x;        // pixels < 2^31 (checked by the caller)
+    if (p >= a.B * a.H * a.W) return;
+    const int xx = p % a.W, yy = (p / a.W) % a.H, b = p / (a.W * a.H);
+    float xin[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = a.flip ? 1 - tap / 3 : tap / 3 - 1, dx = a.flip ? 1 - tap % 3 : tap % 3 - 1;
+        const int ys = yy + dy, xs = xx + dx;
+        xin[tap] = (ys >= 0 && ys < a.H && xs >= 0 && xs < a.W) ? bf2f(a.x[((size_t)b * a.H + ys) * a.W + xs]) : 0.f;
+    }
+    for (int g = 0; g < a.Cout / 8; ++g) {
+        V8 o;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) o.v[i] = a.bias ? a.bias[8 * g + i] : 0.f;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const float4 w0 = *reinterpret_cast<const float4 *>(&sw[tap * a.Cout + 8 * g]);
+            const float4 w1 = *reinterpret_cast<const float4 *>(&sw[tap * a.Cout + 8 * g + 4]);
+            o.v[0] += w0.x * xin[tap]; o.v[1] += w0.y * xin[tap]; o.v[2] += w0.z * xin[tap]; o.v[3] += w0.w * xin[tap];
+            o.v[4] += w1.x * xin[tap]; o.v[5] += w1.y * xin[tap]; o.v[6] += w1.z * xin[tap]; o.v[7] += w1.w * xin[tap];
+        }
+        if (a.relu)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) o.v[i] = o.v[i] < 0.f ? 0.f : o.v[i];
+        st8(a.y + (size_t)p * a.Cout + 8 * g, o);
+    }
+}
+
+// 3x3 convolution from Cin (a multiple of 32) channels to one: an 8 x 32
+// pixel tile per block of 256 threads, one pixel per thread; per chunk of 32
+// channels the tile and its halo are staged in LDS as in k_conv32, the 9 x 32
+// weights as floats.  Load-bound: 2 Cin B in and 2 (+4) B out per pixel.
+__global__ void __launch_bounds__(kB) k_conv_to1(ConvArgs a)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sX[kIH * kIW * kPS];
+    __shared__ float sw[9 * 32];
+    const int tilesX = (a.W + kTW - 1) / kTW, tilesY = (a.H + kTH - 1) / kTH;
+    int tile = blockIdx.x;
+    const int tx = tile % tilesX;
+    tile /= tilesX;
+    const int ty = tile % tilesY;
+    const int bimg = tile / tilesY;
+    const int x0 = tx * kTW, y0 = ty * kTH;
+    const int ry0 = threadIdx.x / kTW, rx0 = threadIdx.x % kTW;
+    const int Cin = a.Cin;
+    float acc = 0.f;
+    for (int c0 = 0; c0 < Cin; c0 += 32) {
+        __syncthreads();
+        for (int s = threadIdx.x; s < kIH * kIW * 4; s += kB) {
+            const int pix = s >> 2, seg = s & 3;
+            const int iy = pix / kIW, ix = pix - iy * kIW;
+            const int ys = y0 + iy - 1, xs = x0 + ix - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (ys >= 0 && ys < a.H && xs >= 0 && xs < a.W)
+                v = *reinterpret_cast<const uint4 *>(a.x + ((size_t)(bimg * a.H + ys) * a.W + xs) * Cin + c0 + 8 * seg);
+            *reinterpret_cast<uint4 *>(&sX[pix * kPS + 8 * seg]) = v;
+        }
+        for (int k = threadIdx.x; k < 9 * 32; k += kB) sw[k] = bf2f(a.w[(k >> 5) * Cin + c0 + (k & 31)]);
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = a.flip ? 1 - tap / 3 : tap / 3 - 1, dx = a.flip ? 1 - tap % 3 : tap % 3 - 1;
+            const bf16 *q = &sX[((ry0 + 1 + dy) * kIW + rx0 + 1 + dx) * kPS];
+#pragma unroll
+            for (int seg = 0; seg < 4; ++seg) {
+                const V8 v = ld8(q + 8 * seg);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc += sw[tap * 32 + 8 * seg + i] * v.v[i];
+            }
+        }
+    }
+    const int yy = y0 + ry0, xx = x0 + rx0;
+    if (yy >= a.H || xx >= a.W) return;
+    const size_t p = ((size_t)bimg * a.H + yy) * a.W + xx;
+    float v = acc + (a.bias ? a.bias[0] : 0.f);
+    if (a.relu && v < 0.f) v = 0.f;
+    if (a.y) a.y[p] = f2bf(v);
+    if (a.yf) a.yf[p] = v;
+}
+
+// k_conv32p where the channels allow it (Cin a multiple of 32, Cout of 32);
+// the one-channel input and output layers on their load-bound kernels;
+// otherwise k_conv with the widest channel blocking that divides CoutP
+// (small Cin takes the flattened path, one tile)
 void launch_conv(const ConvArgs &a, hipStream_t s)
 {
+    if (a.Cin % 32 == 0 && a.Cout % 32 == 0 && a.CoutP == a.Cout) {
+        // one persistent workgroup per CU and occupancy slot
+        static int slots4 = 0, slots2 = 0;
+        if (!slots4) {
+            int dev = 0, cus = 0, o4 = 0, o2 = 0;
+            HU_CHECK(hipGetDevice(&dev));
+            HU_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            HU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, k_conv32p<4, 1>, kPB, 0));
+            HU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, k_conv32p<2, 2>, kPB, 0));
+            slots2 = cus * std::max(o2, 1);
+            slots4 = cus * std::max(o4, 1);
+        }
+        const int mf = a.Cout % 64 == 0 ? 4 : 2, nCo = a.Cout / (16 * mf);
+        const int G = std::max(1, std::min(conv32p_tiles(a), (mf == 4 ? slots4 : slots2) / nCo));
+        if (mf == 4) k_conv32p<4, 1><<<G * nCo, kPB, 0, s>>>(a);
+        else k_conv32p<2, 2><<<G * nCo, kPB, 0, s>>>(a);
+        return;
+    }
+    if (a.Cin == 1 && a.Cout % 8 == 0 && a.Cout <= 256 && a.y && (double)a.B * a.H * a.W < 2e9) {
+        const int n = a.B * a.H * a.W;
+        k_conv_c1<<<(n + kB - 1) / kB, kB, 0, s>>>(a);
+        return;
+    }
+    if (a.Cout == 1 && a.Cin % 32 == 0) {
+        k_conv_to1<<<a.B * ((a.H + kTH - 1) / kTH) * ((a.W + kTW - 1) / kTW), kB, 0, s>>>(a);
+        return;
+    }
     const int P = a.B * a.H * a.W;
     const int mt = (a.Cin & 15) ? 1 : (a.CoutP % 64 == 0) ? 4 : (a.CoutP % 32 == 0) ? 2 : 1;
     const dim3 grid((P + 255) / 256, a.CoutP / (16 * mt));
@@ -318,7 +642,99 @@ __global__ void __launch_bounds__(64) k_conv_wgrad16(const bf16 *dy, const bf16 
     }
 }
 
-// bias gradient: db[co] += sum_p dY[p][co]
+// Weight gradient of the one-channel input and output layers (Cin = 1 or
+// Cout = 1, Cout 9 Cin <= 512 weights): thread o of a block owns weights o and
+// o + 256 (dW[co][tap][ci] in the order of the weight layout) and sums them over
+// the 8 x 32 pixel tiles blockIdx.x, + gridDim.x, ...; per tile dY and the input
+// with its halo are staged in LDS as floats.  One atomic per weight and block.
+constexpr int kSmallC = 32;      // widest channel count of the staged operand
+
+__global__ void __launch_bounds__(kB) k_wgrad_small(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
+                                                    int Cin, int Cout)
+{
+    __shared__ float sD[kTH * kTW * kSmallC];
+    __shared__ float sXs[kIH * kIW * kSmallC];
+    const int tilesX = (W + kTW - 1) / kTW, tilesY = (H + kTH - 1) / kTH, nTiles = B * tilesX * tilesY;
+    const int N = Cout * 9 * Cin;
+    int co[2], off[2];
+    bool own[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int o = threadIdx.x + kB * j;
+        own[j] = o < N;
+        const int oo = own[j] ? o : 0;
+        co[j] = oo / (9 * Cin);
+        const int tap = (oo / Cin) % 9, ci = oo % Cin;
+        off[j] = ((tap / 3) * kIW + tap % 3) * Cin + ci;   // input offset of the tap from the pixel's top-left halo
+    }
+    float acc[2] = {0.f, 0.f};
+    for (int t = blockIdx.x; t < nTiles; t += gridDim.x) {
+        int tt = t;
+        const int tx = tt % tilesX;
+        tt /= tilesX;
+        const int ty = tt % tilesY;
+        const int b = tt / tilesY;
+        const int x0 = tx * kTW, y0 = ty * kTH;
+        __syncthreads();
+        for (int k = threadIdx.x; k < kTH * kTW * Cout; k += kB) {
+            const int px = k / Cout, c = k - px * Cout;
+            const int yy = y0 + px / kTW, xx = x0 + px % kTW;
+            sD[k] = (yy < H && xx < W) ? bf2f(dy[(((size_t)b * H + yy) * W + xx) * Cout + c]) : 0.f;
+        }
+        for (int k = threadIdx.x; k < kIH * kIW * Cin; k += kB) {
+            const int px = k / Cin, c = k - px * Cin;
+            const int ys = y0 + px / kIW - 1, xs = x0 + px % kIW - 1;
+            sXs[k] = (ys >= 0 && ys < H && xs >= 0 && xs < W) ? bf2f(x[(((size_t)b * H + ys) * W + xs) * Cin + c]) : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (!own[j]) continue;
+            float s = 0.f;
+#pragma unroll 1
+            for (int py = 0; py < kTH; ++py)
+#pragma unroll 8
+                for (int px = 0; px < kTW; ++px)
+                    s += sD[(py * kTW + px) * Cout + co[j]] * sXs[(py * kIW + px) * Cin + off[j]];
+            acc[j] += s;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (own[j]) atomicAdd(&dw[threadIdx.x + kB * j], acc[j]);
+}
+
+// Elementwise kernels take 8 channels (one 16-byte vector) per thread: every
+// activation they touch has a channel count that is a multiple of 8.
+
+// bias gradient: db[co] += sum_p dY[p][co].  Thread t of a block reads the 8
+// channels 8 (t % (C/8)) of pixels t / (C/8) + k 256 / (C/8); the partial
+// sums of one channel group meet in LDS, one atomic per channel and block.
+// C a multiple of 8 dividing 2048.
+__global__ void __launch_bounds__(kB) k_bias_grad8(const bf16 *dy, float *db, int P, int C)
+{
+    __shared__ float sh[kB][9];
+    const int groups = C / 8, cg = threadIdx.x % groups, row = threadIdx.x / groups, rows = kB / groups;
+    V8 s;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s.v[i] = 0.f;
+    for (int p = blockIdx.x * rows + row; p < P; p += gridDim.x * rows) {
+        const V8 v = ld8(dy + (size_t)p * C + 8 * cg);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s.v[i] += v.v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sh[threadIdx.x][i] = s.v[i];
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        const int g = threadIdx.x / 8, i = threadIdx.x % 8;
+        float t = 0.f;
+        for (int r = 0; r < rows; ++r) t += sh[r * groups + g][i];
+        atomicAdd(&db[threadIdx.x], t);
+    }
+}
+
+// bias gradient for any C (the 1-channel output layer): one block per channel
 __global__ void k_bias_grad(const bf16 *dy, float *db, int P, int C)
 {
     __shared__ float sh[kB];
@@ -334,72 +750,106 @@ __global__ void k_bias_grad(const bf16 *dy, float *db, int P, int C)
     if (threadIdx.x == 0) atomicAdd(&db[c], sh[0]);
 }
 
-// dPre = dY * (Y > 0)
-__global__ void k_relu_grad(const bf16 *dy, const bf16 *y, bf16 *out, size_t n)
+// dPre = dY * (Y > 0); n8 = elements / 8
+__global__ void k_relu_grad(const bf16 *dy, const bf16 *y, bf16 *out, size_t n8)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    if (k < n) out[k] = bf2f(y[k]) > 0.f ? dy[k] : (bf16)0;
+    if (k >= n8) return;
+    const uint4 g = reinterpret_cast<const uint4 *>(dy)[k], a = reinterpret_cast<const uint4 *>(y)[k];
+    const uint32_t gw[4] = {g.x, g.y, g.z, g.w}, aw[4] = {a.x, a.y, a.z, a.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t lo = __uint_as_float(aw[i] << 16) > 0.f ? 0x0000FFFFu : 0u;
+        const uint32_t hi = __uint_as_float(aw[i] & 0xFFFF0000u) > 0.f ? 0xFFFF0000u : 0u;
+        o[i] = gw[i] & (lo | hi);
+    }
+    reinterpret_cast<uint4 *>(out)[k] = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// 2x2 average pool (NHWC); H, W even
+// 2x2 average pool (NHWC); H, W even; one thread per output pixel and 8 channels
 __global__ void k_pool(const bf16 *x, bf16 *y, int B, int H, int W, int C)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    const int Ho = H / 2, Wo = W / 2;
-    if (k >= (size_t)B * Ho * Wo * C) return;
-    const int c = (int)(k % C);
-    size_t r = k / C;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    if (k >= (size_t)B * Ho * Wo * C8) return;
+    const int c = (int)(k % C8) * 8;
+    size_t r = k / C8;
     const int xo = (int)(r % Wo); r /= Wo;
     const int yo = (int)(r % Ho);
     const int b = (int)(r / Ho);
-    auto at = [&](int yy, int xx) { return bf2f(x[(((size_t)b * H + yy) * W + xx) * C + c]); };
-    y[k] = f2bf(0.25f * (at(2 * yo, 2 * xo) + at(2 * yo, 2 * xo + 1) + at(2 * yo + 1, 2 * xo) + at(2 * yo + 1, 2 * xo + 1)));
+    const bf16 *q = x + (((size_t)b * H + 2 * yo) * W + 2 * xo) * C + c;
+    const V8 a = ld8(q), bb = ld8(q + C), cc = ld8(q + (size_t)W * C), d = ld8(q + (size_t)W * C + C);
+    V8 o;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] = 0.25f * (a.v[i] + bb.v[i] + cc.v[i] + d.v[i]);
+    st8(y + k * 8, o);
 }
 
 __global__ void k_pool_grad(const bf16 *dy, bf16 *dx, int B, int H, int W, int C)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    if (k >= (size_t)B * H * W * C) return;
-    const int c = (int)(k % C);
-    size_t r = k / C;
+    const int C8 = C / 8;
+    if (k >= (size_t)B * H * W * C8) return;
+    const int c = (int)(k % C8) * 8;
+    size_t r = k / C8;
     const int xx = (int)(r % W); r /= W;
     const int yy = (int)(r % H);
     const int b = (int)(r / H);
-    dx[k] = f2bf(0.25f * bf2f(dy[(((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]));
+    V8 o = ld8(dy + (((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] *= 0.25f;
+    st8(dx + k * 8, o);
 }
 
 // nearest x2 upsample plus skip: y = up(x) + s
 __global__ void k_up_add(const bf16 *x, const bf16 *s, bf16 *y, int B, int H, int W, int C)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    if (k >= (size_t)B * H * W * C) return;
-    const int c = (int)(k % C);
-    size_t r = k / C;
+    const int C8 = C / 8;
+    if (k >= (size_t)B * H * W * C8) return;
+    const int c = (int)(k % C8) * 8;
+    size_t r = k / C8;
     const int xx = (int)(r % W); r /= W;
     const int yy = (int)(r % H);
     const int b = (int)(r / H);
-    y[k] = f2bf(bf2f(x[(((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]) + bf2f(s[k]));
+    V8 o = ld8(x + (((size_t)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c);
+    const V8 t = ld8(s + k * 8);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] += t.v[i];
+    st8(y + k * 8, o);
 }
 
 // gradient of up(x): dx[y][x] = sum of the 2x2 block of dy
 __global__ void k_up_grad(const bf16 *dy, bf16 *dx, int B, int Hs, int Ws, int C)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    if (k >= (size_t)B * Hs * Ws * C) return;
-    const int c = (int)(k % C);
-    size_t r = k / C;
+    const int C8 = C / 8;
+    if (k >= (size_t)B * Hs * Ws * C8) return;
+    const int c = (int)(k % C8) * 8;
+    size_t r = k / C8;
     const int xx = (int)(r % Ws); r /= Ws;
     const int yy = (int)(r % Hs);
     const int b = (int)(r / Hs);
-    const int H = 2 * Hs, W = 2 * Ws;
-    auto at = [&](int y2, int x2) { return bf2f(dy[(((size_t)b * H + y2) * W + x2) * C + c]); };
-    dx[k] = f2bf(at(2 * yy, 2 * xx) + at(2 * yy, 2 * xx + 1) + at(2 * yy + 1, 2 * xx) + at(2 * yy + 1, 2 * xx + 1));
+    const int W = 2 * Ws;
+    const bf16 *q = dy + (((size_t)b * 2 * Hs + 2 * yy) * W + 2 * xx) * C + c;
+    const V8 a = ld8(q), bb = ld8(q + C), cc = ld8(q + (size_t)W * C), d = ld8(q + (size_t)W * C + C);
+    V8 o;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] = a.v[i] + bb.v[i] + cc.v[i] + d.v[i];
+    st8(dx + k * 8, o);
 }
 
-__global__ void k_add_bf(const bf16 *a, const bf16 *b, bf16 *y, size_t n)
+// y = a + b; n8 = elements / 8
+__global__ void k_add_bf(const bf16 *a, const bf16 *b, bf16 *y, size_t n8)
 {
     const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    if (k < n) y[k] = f2bf(bf2f(a[k]) + bf2f(b[k]));
+    if (k >= n8) return;
+    V8 o = ld8(a + 8 * k);
+    const V8 t = ld8(b + 8 * k);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] += t.v[i];
+    st8(y + 8 * k, o);
 }
 
 __global__ void k_to_bf(const float *x, bf16 *y, size_t n, float scale)
@@ -540,12 +990,21 @@ void conv_wgrad(csrst_warmstart *u, int li, const bf16 *dpre, const bf16 *x, int
     const int P = B * H * W;
     const int chunk = 1024;
     dim3 grid((9 * l.cin + 63) / 64, pad16(l.cout) / 16, (P + chunk - 1) / chunk);
-    if (l.cin % 16 == 0 && l.cout % 16 == 0)
+    const bool small = (l.cin == 1 && l.cout <= kSmallC) || (l.cout == 1 && l.cin <= kSmallC);
+    if (small) {
+        const int tiles = B * ((H + kTH - 1) / kTH) * ((W + kTW - 1) / kTW);
+        k_wgrad_small<<<std::min(tiles, 1024), kB, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout);
+    } else if (l.cin % 16 == 0 && l.cout % 16 == 0)
         k_conv_wgrad16<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
     else
         k_conv_wgrad<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
-    dim3 bg(std::min(64, (P + kB - 1) / kB), l.cout);
-    k_bias_grad<<<bg, kB, 0, u->s>>>(dpre, u->g.p + l.boff, P, l.cout);
+    if (l.cout % 8 == 0 && kB % (l.cout / 8) == 0) {
+        const int rows = kB / (l.cout / 8);
+        k_bias_grad8<<<std::min(1024, (P + rows - 1) / rows), kB, 0, u->s>>>(dpre, u->g.p + l.boff, P, l.cout);
+    } else {
+        dim3 bg(std::min(64, (P + kB - 1) / kB), l.cout);
+        k_bias_grad<<<bg, kB, 0, u->s>>>(dpre, u->g.p + l.boff, P, l.cout);
+    }
 }
 
 void refresh_bf16(csrst_warmstart *u)
@@ -600,11 +1059,11 @@ void forward(csrst_warmstart *u, int B)
     auto P = [&](int i) { return u->ptr(A[i]); };
     auto pool = [&](int src, int dst) {
         const auto &d = A[dst];
-        k_pool<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(P(src), P(dst), B, A[src].h, A[src].w, d.c);
+        k_pool<<<nb((size_t)B * d.h * d.w * d.c / 8), kB, 0, u->s>>>(P(src), P(dst), B, A[src].h, A[src].w, d.c);
     };
     auto upadd = [&](int src, int skip, int dst) {
         const auto &d = A[dst];
-        k_up_add<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(P(src), P(skip), P(dst), B, d.h, d.w, d.c);
+        k_up_add<<<nb((size_t)B * d.h * d.w * d.c / 8), kB, 0, u->s>>>(P(src), P(skip), P(dst), B, d.h, d.w, d.c);
     };
     conv(u, 0, P(0), P(1), nullptr, B, H, W);
     conv(u, 1, P(1), P(2), nullptr, B, H, W);
@@ -646,7 +1105,7 @@ void backward(csrst_warmstart *u, int B)
         const size_t n = (size_t)B * h * w * l.cout;
         const bf16 *dpre = D(yout);
         if (l.relu) {
-            k_relu_grad<<<nb(n), kB, 0, u->s>>>(D(yout), P(yout), pre, n);
+            k_relu_grad<<<nb(n / 8), kB, 0, u->s>>>(D(yout), P(yout), pre, n / 8);
             dpre = pre;
         }
         conv_wgrad(u, li, dpre, P(xin), B, h, w);
@@ -655,15 +1114,15 @@ void backward(csrst_warmstart *u, int B)
     auto addgrad = [&](int dst, const bf16 *src) {   // D(dst) += src
         const auto &d = A[dst];
         const size_t n = (size_t)B * d.h * d.w * d.c;
-        k_add_bf<<<nb(n), kB, 0, u->s>>>(D(dst), src, D(dst), n);
+        k_add_bf<<<nb(n / 8), kB, 0, u->s>>>(D(dst), src, D(dst), n / 8);
     };
     auto upgrad = [&](int dstLow, int srcHigh) {      // D(dstLow) = up^T D(srcHigh)
         const auto &d = A[dstLow];
-        k_up_grad<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(D(srcHigh), D(dstLow), B, d.h, d.w, d.c);
+        k_up_grad<<<nb((size_t)B * d.h * d.w * d.c / 8), kB, 0, u->s>>>(D(srcHigh), D(dstLow), B, d.h, d.w, d.c);
     };
     auto poolgrad = [&](bf16 *dst, int srcLow, int hiIdx) {   // dst = pool^T D(srcLow)
         const auto &d = A[hiIdx];
-        k_pool_grad<<<nb((size_t)B * d.h * d.w * d.c), kB, 0, u->s>>>(D(srcLow), dst, B, d.h, d.w, d.c);
+        k_pool_grad<<<nb((size_t)B * d.h * d.w * d.c / 8), kB, 0, u->s>>>(D(srcLow), dst, B, d.h, d.w, d.c);
     };
     bf16 *tmp = u->ptr(G[23]);
     back(14, 20, 21, H, W, true);      // out -> D20
