@@ -7,12 +7,13 @@
 // the multigrid still iterates to its tolerance.
 //
 // Layout: activations NHWC (channels fastest), weights [Cout][3][3][Cin].
-// Every 3x3 convolution (stride 1, zero padding 1) is an implicit GEMM on
-// v_mfma_f32_16x16x16bf16_1k: a wavefront computes a 16 (output channel) x 64
-// (pixel) tile as four 16x16 accumulators; K runs over (tap, input channel).
-// The same kernel, with transposed weights and mirrored taps, gives the
-// input gradient; the weight gradient is a GEMM over pixels split across
-// workgroups with FP32 atomics.
+// The 3x3 convolutions (stride 1, zero padding 1) with 32 or more channels on
+// both sides are a persistent implicit GEMM on v_mfma_f32_16x16x32_bf16 with
+// LDS-staged input tiles and weights (k_conv32p); K runs over (tap, input
+// channel).  The same kernel, with transposed weights and mirrored taps, gives
+// the input gradient.  The one-channel input and output layers are load-bound
+// and run on plain vector kernels.  The weight gradient is a GEMM over pixels
+// split across workgroups with FP32 atomics (16x16x16 MFMA).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
