@@ -135,6 +135,95 @@ def warmstart_bench(H=1024, W=2048, B=4, warmup=2, reps=5):
                 train_pairs_per_s=B / tt, train_wall_tflops=3 * sum(per_layer) / tt / 1e12)
 
 
+def warmstart_density(rng, H, W, h):
+    """One density of the learned-start workload: 1-4 Gaussians, sigma
+    20-500 um, centres in the middle 40 % of each axis, a chirp (the x offset
+    scaled by 1 + c (y - y0) / H, |c| <= 0.3), amplitudes summing to one; the
+    grid edges are the grounded chamber wall."""
+    import numpy as np
+    rho = np.zeros((H, W))
+    k = int(rng.integers(1, 5))
+    amps = rng.random(k)
+    amps /= amps.sum()
+    for a in amps:
+        s = max(rng.uniform(20e-6, 500e-6) / h, 0.5)
+        cx, cy = rng.uniform(0.3 * W, 0.7 * W), rng.uniform(0.3 * H, 0.7 * H)
+        chirp = rng.uniform(-0.3, 0.3)
+        r = int(np.ceil(7 * s * 1.3)) + 2
+        x0, x1 = max(int(cx) - r, 1), min(int(cx) + r, W - 1)
+        y0, y1 = max(int(cy) - r, 1), min(int(cy) + r, H - 1)
+        y, x = np.mgrid[y0:y1, x0:x1]
+        xs = (x - cx) * (1 + chirp * (y - cy) / H)
+        rho[y0:y1, x0:x1] += a * np.exp(-(xs ** 2 + (y - cy) ** 2) / (2 * s * s)) / (2 * np.pi * s * s)
+    return rho
+
+
+def warmstart_experiment(H=1024, W=2048, h=10e-6, train_seconds=300.0, test=1000, kmax=12, nrhs=2, seed=11,
+                         load=None, save=None):
+    """The learned start on 2048 x 1024 densities, network between V-cycles:
+    (1) train on (residual, error) pairs of the multigrid's own iterates, the
+    first half of the time on plain V-cycle iterates, the second on the
+    iterates of the learned loop; (2) on `test` held-out densities solve to a
+    relative residual of 1e-10 plainly from zero, with the network's step
+    before every V-cycle, and with it before the first only; report V-cycles,
+    final residuals and wall time per solve (host copies included).  load:
+    a weight file (numpy .npy) to start from; save: where to write the
+    trained weights."""
+    import numpy as np
+    from csrst_amd import poisson as PS
+    from csrst_amd.warmstart import WarmStart
+    cls = np.zeros((H, W), np.uint8)
+    cls[1:-1, 1:-1] = PS.UNKNOWN
+    u = WarmStart(H, W, max_batch=4)
+    if load:
+        u.set_weights(np.load(load))
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    losses, pairs, calls = [], 0, 0
+    while time.perf_counter() - t0 < train_seconds:
+        el = (time.perf_counter() - t0) / train_seconds
+        b = np.stack([warmstart_density(rng, H, W, h) for _ in range(nrhs)])
+        lr = 2e-4 if el < 0.6 else 1e-4 if el < 0.85 else 4e-5
+        loss, n = u.train_iterates(h, h, cls, b, kmax=kmax, inloop=el >= 0.5, lr=lr)
+        losses.append(loss)
+        pairs += n
+        calls += 1
+        if calls % 20 == 0:
+            print("warmstart: %.0f s, %d pairs, loss %.4f (last 20 calls)" % (
+                time.perf_counter() - t0, pairs, float(np.mean(losses[-20:]))), file=sys.stderr, flush=True)
+    ttrain = time.perf_counter() - t0
+    if save:
+        np.save(save, u.weights())
+    trng = np.random.default_rng(seed + 1000003)
+    res = {0: [], 1: [], 2: []}
+    for i in range(test):
+        b = warmstart_density(trng, H, W, h)
+        for mode in (0, 1, 2):
+            t1 = time.perf_counter()
+            _, c, r = u.solve(h, h, cls, b, tol=1e-10, maxit=100, mode=mode)
+            res[mode].append((c, r, time.perf_counter() - t1))
+        if (i + 1) % 100 == 0:
+            print("warmstart: %d held-out densities, mean V-cycles plain %.2f, in-loop %.2f, one-shot %.2f" % (
+                i + 1, np.mean([x[0] for x in res[0]]), np.mean([x[0] for x in res[1]]),
+                np.mean([x[0] for x in res[2]])), file=sys.stderr, flush=True)
+    u.close()
+
+    def summary(v):
+        c = np.array([x[0] for x in v], float)
+        r = np.array([x[1] for x in v], float)
+        t = np.array([x[2] for x in v], float)
+        return dict(vcycles_mean=float(c.mean()), vcycles_median=float(np.median(c)), vcycles_max=int(c.max()),
+                    resid_max=float(r.max()), resid_median=float(np.median(r)), ms_per_solve=1e3 * float(t.mean()))
+    out = dict(grid="%dx%d" % (W, H), h_m=h, tol=1e-10, start=load or "He initialisation", train_seconds=ttrain,
+               train_calls=calls, train_pairs=pairs, loss_first=float(np.mean(losses[:20])) if losses else None,
+               loss_last=float(np.mean(losses[-20:])) if losses else None, test_densities=test,
+               plain=summary(res[0]), inloop=summary(res[1]), oneshot=summary(res[2]))
+    out["vcycle_ratio_inloop"] = out["plain"]["vcycles_mean"] / out["inloop"]["vcycles_mean"]
+    out["vcycle_ratio_oneshot"] = out["plain"]["vcycles_mean"] / out["oneshot"]["vcycles_mean"]
+    out["time_ratio_inloop"] = out["plain"]["ms_per_solve"] / out["inloop"]["ms_per_solve"]
+    return out
+
+
 def _oracle_worker(seconds):
     """Child process of cpu_baseline's all-cores leg: one sample, as JSON."""
     print(json.dumps(_oracle_sample(seconds)), flush=True)
@@ -233,6 +322,13 @@ def main():
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
     ap.add_argument("--warmstart", action="store_true",
                     help="only time the warm-start U-Net inference on 2048x1024 densities and print it")
+    ap.add_argument("--warmstart-experiment", action="store_true",
+                    help="only run the learned-start experiment (network between V-cycles, 2048x1024) and print it")
+    ap.add_argument("--ws-train-seconds", type=float, default=300.0)
+    ap.add_argument("--ws-test", type=int, default=1000, help="held-out densities of the experiment")
+    ap.add_argument("--ws-grid", default="2048x1024", help="W x H of the experiment")
+    ap.add_argument("--ws-load", default=None, help="weights (.npy) to start the experiment from")
+    ap.add_argument("--ws-save", default=None, help="write the trained weights (.npy) here")
     ap.add_argument("--parity-mode", action="store_true",
                     help="only time the two-grid FP64 step on the CPU baseline's workload and print it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -256,6 +352,11 @@ def main():
         return
     if a.warmstart:
         print(json.dumps(warmstart_bench()), flush=True)
+        return
+    if a.warmstart_experiment:
+        gw, gh = (int(v) for v in a.ws_grid.split("x"))
+        print(json.dumps(warmstart_experiment(H=gh, W=gw, h=10e-6 * 2048 / gw, train_seconds=a.ws_train_seconds,
+                                              test=a.ws_test, load=a.ws_load, save=a.ws_save)), flush=True)
         return
     cfg = config(a.config)
     n = a.particles or cfg["n"]

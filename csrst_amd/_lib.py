@@ -37,7 +37,8 @@ SYMBOLS = [
     "csrst_sf_charge_density_diff",
     "csrst_warmstart_create", "csrst_warmstart_destroy", "csrst_warmstart_num_params",
     "csrst_warmstart_get_weights", "csrst_warmstart_set_weights", "csrst_warmstart_infer",
-    "csrst_warmstart_gradients", "csrst_warmstart_train_step",
+    "csrst_warmstart_gradients", "csrst_warmstart_train_step", "csrst_warmstart_solve",
+    "csrst_warmstart_train_iterates",
 ]
 
 

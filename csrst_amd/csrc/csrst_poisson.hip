@@ -396,6 +396,22 @@ double Solver::residual(const double *b, const double *phi)
     return norm_res(0);
 }
 
+double Solver::begin(const double *b, const double *phi0)
+{
+    prepare(b);
+    const Level &L = lv_[0];
+    k_load_guess<<<nblk((size_t)L.nx * L.ny), kBlock, 0, s_>>>(phi0, L);
+    HU_LAUNCH_CHECK();
+    return bnorm_;
+}
+
+void Solver::store(const double *b, double *phi)
+{
+    const size_t m = (size_t)nx_ * ny_;
+    k_store<<<nblk(m), kBlock, 0, s_>>>(cls_, b, lv_[0].phi, phi, m);
+    HU_LAUNCH_CHECK();
+}
+
 Stats Solver::solve(const double *b, double *phi, double tol, int maxit)
 {
     Stats st;

@@ -24,6 +24,7 @@
 
 #include "csrst_hip.h"
 #include "hip_util.h"
+#include "poisson_mg.h"
 
 using hu::DBuf;
 using hu::Error;
@@ -938,6 +939,19 @@ struct csrst_warmstart {
     std::vector<Buf> A;            // forward activations
     std::vector<Buf> Gd;           // gradient buffers (same shapes)
     int batch = 0;
+    // the multigrid of the grid H x W and the buffers of the network-in-the-loop
+    // solve and of its training pairs
+    struct Loop {
+        pmg::Solver mg;
+        bool ready = false;
+        double hx = 0, hy = 0;
+        std::vector<uint8_t> cls;      // host copy of the node classes of the hierarchy
+        size_t unknowns = 0;
+        DBuf<uint8_t> dcls;
+        DBuf<double> b, phi, pstar, part;
+        DBuf<bf16> pairIn;             // [maxBatch][H][W] r / rms(r)
+        DBuf<float> pairTgt;           // [maxBatch][H][W] e / rms(e)
+    } loop;
 
     bf16 *ptr(const Buf &b) { return act.p + b.off; }
 };
@@ -1047,9 +1061,10 @@ void plan(csrst_warmstart *u, int B)
     for (auto &e : sh) big = std::max(big, (size_t)B * e[0] * e[1] * e[2]);
     u->Gd.push_back({off, 0, 0, 0}); off += (big + 63) / 64 * 64;
     u->Gd.push_back({off, 0, 0, 0}); off += (big + 63) / 64 * 64;
-    u->act.alloc(off);
-    u->outf.alloc((size_t)B * H * W);
-    u->tgt.alloc((size_t)B * H * W);
+    // grow only: a smaller batch reuses the arena with its own offsets
+    if (off > u->act.n) u->act.alloc(off);
+    if ((size_t)B * H * W > u->outf.n) u->outf.alloc((size_t)B * H * W);
+    if ((size_t)B * H * W > u->tgt.n) u->tgt.alloc((size_t)B * H * W);
     u->batch = B;
 }
 
@@ -1164,12 +1179,201 @@ void upload_input(csrst_warmstart *u, const float *rho, int B, float scale)
     HU_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// The network between V-cycles.  At an iterate phi_k of the multigrid the
+// residual r_k = b - A phi_k, scaled to unit RMS over the unknowns, goes
+// through the network; its output c is taken as the direction of the error
+// e_k = A^-1 r_k and the iterate moves by the step that minimises the energy
+// norm of the error along it: phi += alpha c, alpha = (c . r) / (c . A c).  A
+// V-cycle follows.  Training pairs are (r_k / rms(r_k), e_k / rms(e_k)) over
+// the iterates of the solver itself, with e_k from a solve to round-off.
+
+constexpr int kRed = 256;      // blocks of the fixed-order reductions
+
+// r * inv as bf16 network input, zero off the unknowns
+__global__ void k_res_in(const double *res, const uint8_t *act, bf16 *in, size_t n, double inv)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) in[k] = act[k] ? f2bf((float)(res[k] * inv)) : (bf16)0;
+}
+
+// (pstar - phi) * inv as float target, zero off the unknowns
+__global__ void k_err_tgt(const double *pstar, const double *phi, const uint8_t *act, float *t, size_t n, double inv)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) t[k] = act[k] ? (float)((pstar[k] - phi[k]) * inv) : 0.f;
+}
+
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *part)
+{
+    __shared__ double sa[kB], sb[kB];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = kB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa[0];
+        part[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+// partial sums of c . r and c . A c over the unknowns (A the level's 5-point
+// operator with the unknowns' couplings only, as the multigrid applies it)
+__global__ void __launch_bounds__(kB) k_step_dots(pmg::Level L, const float *c, double *part)
+{
+    double cr = 0.0, cac = 0.0;
+    const size_t n = (size_t)L.nx * L.ny;
+    for (size_t k = (size_t)blockIdx.x * kB + threadIdx.x; k < n; k += (size_t)gridDim.x * kB) {
+        if (!L.act[k]) continue;
+        const int i = (int)(k % L.nx), j = (int)(k / L.nx);
+        const double ck = c[k];
+        double sx = 0.0, sy = 0.0;
+        if (i > 0 && L.act[k - 1]) sx += c[k - 1];
+        if (i + 1 < L.nx && L.act[k + 1]) sx += c[k + 1];
+        if (j > 0 && L.act[k - L.nx]) sy += c[k - L.nx];
+        if (j + 1 < L.ny && L.act[k + L.nx]) sy += c[k + L.nx];
+        cr += ck * L.res[k];
+        cac += ck * (2.0 * (L.cx + L.cy) * ck - L.cx * sx - L.cy * sy);
+    }
+    block_sum2(cr, cac, part);
+}
+
+// partial sums of (pstar - phi)^2 over the unknowns (second slot unused)
+__global__ void __launch_bounds__(kB) k_err_sumsq(const double *pstar, const double *phi, const uint8_t *act, size_t n,
+                                                 double *part)
+{
+    double s = 0.0, z = 0.0;
+    for (size_t k = (size_t)blockIdx.x * kB + threadIdx.x; k < n; k += (size_t)gridDim.x * kB)
+        if (act[k]) {
+            const double d = pstar[k] - phi[k];
+            s += d * d;
+        }
+    block_sum2(s, z, part);
+}
+
+__global__ void k_sum2_final(double *part, int nb)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nb; ++k) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    part[2 * nb] = a;
+    part[2 * nb + 1] = b;
+}
+
+__global__ void k_axpy_act(pmg::Level L, const float *c, double alpha)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < (size_t)L.nx * L.ny && L.act[k]) L.phi[k] += alpha * (double)c[k];
+}
+
+__global__ void k_copy_d(double *dst, const double *src, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+__global__ void k_zero_d(double *p, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
+
 int ws_fail(csrst_warmstart *u, const std::exception &e)
 {
     const Error *he = dynamic_cast<const Error *>(&e);
     if (u) u->err = e.what();
     hu::global_error() = e.what();
     return he ? he->code : CSRST_EINVAL;
+}
+
+
+// The hierarchy of the grid (hx, hy, cls), set up once and kept while the
+// caller passes the same grid.
+void loop_grid(csrst_warmstart *u, double hx, double hy, const uint8_t *cls)
+{
+    auto &Lp = u->loop;
+    const size_t n = (size_t)u->H * u->W;
+    if (!cls || !(hx > 0) || !(hy > 0)) throw Error(CSRST_EINVAL, "bad grid for the warm-start solve");
+    if (Lp.ready && Lp.hx == hx && Lp.hy == hy && std::equal(Lp.cls.begin(), Lp.cls.end(), cls)) return;
+    size_t unk = 0;
+    for (size_t k = 0; k < n; ++k) {
+        if (cls[k] > pmg::kKnown) throw Error(CSRST_EINVAL, hu::fmt("node class %d at %zu", cls[k], k));
+        unk += cls[k] == pmg::kUnknown;
+    }
+    if (!unk) throw Error(CSRST_EINVAL, "the grid has no unknown nodes");
+    Lp.ready = false;
+    Lp.cls.assign(cls, cls + n);
+    Lp.dcls.alloc(n);
+    Lp.b.alloc(n); Lp.phi.alloc(n); Lp.pstar.alloc(n); Lp.part.alloc(2 * kRed + 2);
+    HU_CHECK(hipMemcpyAsync(Lp.dcls.p, cls, n, hipMemcpyHostToDevice, u->s));
+    Lp.mg.setup(u->W, u->H, hx, hy, Lp.dcls.p, u->s);
+    Lp.hx = hx; Lp.hy = hy; Lp.unknowns = unk;
+    Lp.ready = true;
+}
+
+// the two sums of the last k_step_dots / k_err_sumsq launch on the host
+void loop_sums(csrst_warmstart *u, double &a, double &b)
+{
+    auto &Lp = u->loop;
+    k_sum2_final<<<1, 1, 0, u->s>>>(Lp.part.p, kRed);
+    HU_LAUNCH_CHECK();
+    double h[2];
+    HU_CHECK(hipMemcpyAsync(h, Lp.part.p + 2 * kRed, sizeof h, hipMemcpyDeviceToHost, u->s));
+    HU_CHECK(hipStreamSynchronize(u->s));
+    a = h[0];
+    b = h[1];
+}
+
+// The network's step at the current iterate; rn = ||r|| from resnorm().
+void loop_correct(csrst_warmstart *u, double rn)
+{
+    auto &Lp = u->loop;
+    const pmg::Level &L0 = Lp.mg.level0();
+    const size_t n = (size_t)u->H * u->W;
+    const double rms = rn / std::sqrt((double)Lp.unknowns);
+    if (!(rms > 0)) return;
+    plan(u, 1);
+    k_res_in<<<nb(n), kB, 0, u->s>>>(L0.res, L0.act, u->ptr(u->A[0]), n, 1.0 / rms);
+    forward(u, 1);
+    k_step_dots<<<kRed, kB, 0, u->s>>>(L0, u->outf.p, Lp.part.p);
+    double cr = 0, cac = 0;
+    loop_sums(u, cr, cac);
+    if (cac > 0 && std::isfinite(cr / cac)) {
+        k_axpy_act<<<nb(n), kB, 0, u->s>>>(L0, u->outf.p, cr / cac);
+        HU_LAUNCH_CHECK();
+    }
+}
+
+// One Adam step on the first B staged pairs; returns the loss before it.
+float loop_train(csrst_warmstart *u, int B, float lr)
+{
+    auto &Lp = u->loop;
+    const size_t np = (size_t)B * u->H * u->W;
+    plan(u, B);
+    HU_CHECK(hipMemcpyAsync(u->ptr(u->A[0]), Lp.pairIn.p, np * sizeof(bf16), hipMemcpyDeviceToDevice, u->s));
+    HU_CHECK(hipMemcpyAsync(u->tgt.p, Lp.pairTgt.p, np * sizeof(float), hipMemcpyDeviceToDevice, u->s));
+    forward(u, B);
+    HU_CHECK(hipMemsetAsync(u->lossBuf.p, 0, sizeof(float), u->s));
+    k_mse<<<nb(np), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
+    backward(u, B);
+    ++u->steps;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)u->steps), c2 = 1.f - std::pow(b2, (float)u->steps);
+    k_adam<<<nb(u->nparam), kB, 0, u->s>>>(u->w.p, u->m.p, u->v.p, u->g.p, u->nparam, lr, b1, b2, 1e-8f, c1, c2);
+    refresh_bf16(u);
+    float loss = 0.f;
+    HU_CHECK(hipMemcpyAsync(&loss, u->lossBuf.p, sizeof(float), hipMemcpyDeviceToHost, u->s));
+    HU_CHECK(hipStreamSynchronize(u->s));
+    return loss;
 }
 
 }  // namespace
@@ -1314,6 +1518,121 @@ int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float
         k_adam<<<nb(u->nparam), kB, 0, u->s>>>(u->w.p, u->m.p, u->v.p, u->g.p, u->nparam, lr, b1, b2, 1e-8f, c1, c2);
         refresh_bf16(u);
         HU_CHECK(hipStreamSynchronize(u->s));
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+
+// Poisson solve on the network's H x W node grid (nx = W, ny = H) with the
+// multigrid of csrst_poisson_solve.  mode 0: plain V-cycles from phi; 1: the
+// network's step before every V-cycle; 2: before the first only (a one-shot
+// start).  phi: in the guess, out the solution; cycles and the final relative
+// residual as csrst_poisson_solve reports them.
+int csrst_warmstart_solve(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b, double *phi,
+                          double tol, int maxit, int mode, int *cycles, double *resid)
+{
+    try {
+        if (!u || !b || !phi || maxit < 0 || !(tol >= 0) || mode < 0 || mode > 2)
+            throw Error(CSRST_EINVAL, "bad warm-start solve arguments");
+        loop_grid(u, hx, hy, cls);
+        auto &Lp = u->loop;
+        const size_t n = (size_t)u->H * u->W;
+        HU_CHECK(hipMemcpyAsync(Lp.b.p, b, n * sizeof(double), hipMemcpyHostToDevice, u->s));
+        HU_CHECK(hipMemcpyAsync(Lp.phi.p, phi, n * sizeof(double), hipMemcpyHostToDevice, u->s));
+        const double bn = Lp.mg.begin(Lp.b.p, Lp.phi.p);
+        double r = Lp.mg.resnorm();
+        const double target = tol * (bn > 0 ? bn : 1.0);
+        int cyc = 0, flat = 0;
+        while (r > target && cyc < maxit && flat < 3) {
+            if (mode == 1 || (mode == 2 && cyc == 0)) loop_correct(u, r);
+            Lp.mg.cycle();
+            ++cyc;
+            const double rn = Lp.mg.resnorm();
+            if (!std::isfinite(rn)) throw Error(CSRST_EHIP, "multigrid diverged");
+            flat = rn > 0.95 * r ? flat + 1 : 0;
+            r = rn;
+        }
+        Lp.mg.store(Lp.b.p, Lp.phi.p);
+        HU_CHECK(hipMemcpyAsync(phi, Lp.phi.p, n * sizeof(double), hipMemcpyDeviceToHost, u->s));
+        HU_CHECK(hipStreamSynchronize(u->s));
+        if (cycles) *cycles = cyc;
+        if (resid) *resid = bn > 0 ? r / bn : r;
+        return CSRST_OK;
+    } catch (const std::exception &e) {
+        return ws_fail(u, e);
+    }
+}
+
+// Training on the solver's own iterates.  For each of the nrhs right-hand
+// sides b[nrhs][H][W]: a plain solve to a relative residual of 1e-13 gives
+// phi*; then from phi = 0, at iterates k = 0 .. kmax-1 (until the residual
+// reaches 1e-12), the pair (r_k / rms(r_k), (phi* - phi_k) / rms) is staged
+// and the iterate advances by a V-cycle, preceded by the network's step when
+// inloop is 1 (pairs from the learned loop itself).  Every maxBatch staged
+// pairs make one Adam step of rate lr.  loss: mean loss of those steps;
+// pairs: pairs used.
+int csrst_warmstart_train_iterates(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b,
+                                   int nrhs, int kmax, int inloop, float lr, float *loss, int *pairs)
+{
+    try {
+        if (!u || !b || nrhs < 1 || kmax < 1 || !(lr > 0)) throw Error(CSRST_EINVAL, "bad training arguments");
+        loop_grid(u, hx, hy, cls);
+        auto &Lp = u->loop;
+        const size_t n = (size_t)u->H * u->W;
+        const int MB = u->maxBatch;
+        Lp.pairIn.alloc(std::max(Lp.pairIn.n, (size_t)MB * n));
+        Lp.pairTgt.alloc(std::max(Lp.pairTgt.n, (size_t)MB * n));
+        const pmg::Level &L0 = Lp.mg.level0();
+        int slot = 0, used = 0, nsteps = 0;
+        double lsum = 0.0;
+        for (int i = 0; i < nrhs; ++i) {
+            HU_CHECK(hipMemcpyAsync(Lp.b.p, b + (size_t)i * n, n * sizeof(double), hipMemcpyHostToDevice, u->s));
+            k_zero_d<<<nb(n), kB, 0, u->s>>>(Lp.phi.p, n);
+            HU_LAUNCH_CHECK();
+            const double bn = Lp.mg.begin(Lp.b.p, Lp.phi.p);
+            if (!(bn > 0)) continue;
+            double r = Lp.mg.resnorm();
+            for (int c = 0, flat = 0; r > 1e-13 * bn && c < 200 && flat < 3; ++c) {
+                Lp.mg.cycle();
+                const double rn = Lp.mg.resnorm();
+                flat = rn > 0.95 * r ? flat + 1 : 0;
+                r = rn;
+            }
+            k_copy_d<<<nb(n), kB, 0, u->s>>>(Lp.pstar.p, L0.phi, n);
+            Lp.mg.begin(Lp.b.p, Lp.phi.p);
+            r = Lp.mg.resnorm();
+            for (int k = 0; k < kmax && r > 1e-12 * bn; ++k) {
+                k_err_sumsq<<<kRed, kB, 0, u->s>>>(Lp.pstar.p, L0.phi, L0.act, n, Lp.part.p);
+                double e2 = 0, z = 0;
+                loop_sums(u, e2, z);
+                const double rr = r / std::sqrt((double)Lp.unknowns), re = std::sqrt(e2 / (double)Lp.unknowns);
+                if (rr > 0 && re > 0) {
+                    k_res_in<<<nb(n), kB, 0, u->s>>>(L0.res, L0.act, Lp.pairIn.p + (size_t)slot * n, n, 1.0 / rr);
+                    k_err_tgt<<<nb(n), kB, 0, u->s>>>(Lp.pstar.p, L0.phi, L0.act, Lp.pairTgt.p + (size_t)slot * n, n,
+                                                       1.0 / re);
+                    HU_LAUNCH_CHECK();
+                    if (++slot == MB) {
+                        lsum += loop_train(u, slot, lr);
+                        ++nsteps;
+                        used += slot;
+                        slot = 0;
+                    }
+                }
+                if (inloop) loop_correct(u, r);
+                Lp.mg.cycle();
+                r = Lp.mg.resnorm();
+            }
+        }
+        if (slot) {
+            lsum += loop_train(u, slot, lr);
+            ++nsteps;
+            used += slot;
+        }
+        HU_CHECK(hipStreamSynchronize(u->s));
+        if (loss) *loss = nsteps ? (float)(lsum / nsteps) : 0.f;
+        if (pairs) *pairs = used;
         return CSRST_OK;
     } catch (const std::exception &e) {
         return ws_fail(u, e);

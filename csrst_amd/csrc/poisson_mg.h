@@ -54,6 +54,17 @@ public:
     Stats solve(const double *b, double *phi, double tol, int maxit);
     // residual norm of phi (device) against b without iterating
     double residual(const double *b, const double *phi);
+    // Step-wise use (a caller that acts on the iterate between V-cycles):
+    // begin() loads b and the guess phi0 (device) and returns ||b||; resnorm()
+    // computes the level-0 residual (level0().res) and returns its norm over
+    // the unknowns; cycle() runs one V-cycle on level0().phi; store() writes
+    // the solution (identity rows phi = b) to phi (device).
+    double begin(const double *b, const double *phi0);
+    double resnorm() { return norm_res(0); }
+    void cycle() { vcycle(0); }
+    void store(const double *b, double *phi);
+    const Level &level0() const { return lv_[0]; }
+    hipStream_t stream() const { return s_; }
     int nx() const { return nx_; }
     int ny() const { return ny_; }
     int levels() const { return (int)lv_.size(); }

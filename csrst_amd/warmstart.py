@@ -78,3 +78,38 @@ class WarmStart:
         check(_lib.lib().csrst_warmstart_train_step(self.h, _fp(rho), _fp(phi), C.c_int(len(rho)), C.c_float(scale),
                                                     C.c_float(lr), C.byref(loss)), self.h)
         return loss.value
+
+    def _grid(self, cls):
+        cls = np.ascontiguousarray(cls, np.uint8)
+        if cls.shape != (self.H, self.W):
+            raise ValueError("node classes must be [%d, %d]" % (self.H, self.W))
+        return cls
+
+    def solve(self, hx, hy, cls, b, phi0=None, tol=1e-10, maxit=200, mode=1):
+        """Poisson solve on the network's grid (csrst_warmstart_solve): mode 0
+        plain V-cycles, 1 the network's step before every V-cycle, 2 before the
+        first only.  Returns (phi, cycles, relative residual)."""
+        cls = self._grid(cls)
+        b = np.ascontiguousarray(b, np.float64)
+        if b.shape != cls.shape:
+            raise ValueError("b must be [%d, %d]" % (self.H, self.W))
+        phi = np.zeros_like(b) if phi0 is None else np.array(phi0, np.float64, copy=True, order="C")
+        cyc, res = C.c_int(0), C.c_double(0.0)
+        check(_lib.lib().csrst_warmstart_solve(self.h, C.c_double(hx), C.c_double(hy),
+                                               cls.ctypes.data_as(C.POINTER(C.c_uint8)), _lib.dptr(b), _lib.dptr(phi),
+                                               C.c_double(tol), C.c_int(maxit), C.c_int(mode), C.byref(cyc),
+                                               C.byref(res)), self.h)
+        return phi, cyc.value, res.value
+
+    def train_iterates(self, hx, hy, cls, b, kmax=12, inloop=False, lr=1e-4):
+        """Adam steps on (residual, error) pairs of the solver's own iterates
+        for the right-hand sides b[n, H, W] (csrst_warmstart_train_iterates).
+        Returns (mean loss, pairs used)."""
+        cls = self._grid(cls)
+        b = np.ascontiguousarray(b, np.float64).reshape(-1, self.H, self.W)
+        loss, pairs = C.c_float(0), C.c_int(0)
+        check(_lib.lib().csrst_warmstart_train_iterates(self.h, C.c_double(hx), C.c_double(hy),
+                                                        cls.ctypes.data_as(C.POINTER(C.c_uint8)), _lib.dptr(b),
+                                                        C.c_int(len(b)), C.c_int(kmax), C.c_int(int(inloop)),
+                                                        C.c_float(lr), C.byref(loss), C.byref(pairs)), self.h)
+        return loss.value, pairs.value

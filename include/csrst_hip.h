@@ -326,6 +326,25 @@ int csrst_warmstart_gradients(csrst_warmstart *u, const float *rho, const float 
 /* one Adam step (beta 0.9 / 0.999); *loss is the batch loss before the update */
 int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float *phi, int B, float scale,
                                float lr, float *loss);
+/* The network between V-cycles (no counterpart in the reference either).  The
+ * multigrid of csrst_poisson_solve on the network's grid (nx = W, ny = H),
+ * kept on the device while the caller passes the same hx, hy and cls.  At an
+ * iterate phi_k the residual r_k, scaled to unit RMS over the unknowns, goes
+ * through the network; its output c is a direction for the error and the
+ * iterate moves by alpha c with alpha = (c . r_k) / (c . A c), the step that
+ * minimises the energy norm of the error along c; a V-cycle follows.
+ * mode 0: plain V-cycles from phi; 1: the network's step before every V-cycle;
+ * 2: before the first V-cycle only.  b, phi: [H][W] doubles (host); phi in:
+ * the guess, out: the solution; cycles and resid as csrst_poisson_solve. */
+int csrst_warmstart_solve(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b,
+                          double *phi, double tol, int maxit, int mode, int *cycles, double *resid);
+/* Training on the solver's own iterates: for each of the nrhs right-hand sides
+ * b[nrhs][H][W], phi* from a solve to 1e-13, then pairs (r_k / rms(r_k),
+ * (phi* - phi_k) / rms(phi* - phi_k)) at iterates k < kmax of the loop (with
+ * the network's step when inloop is 1), one Adam step of rate lr per maxBatch
+ * pairs.  *loss: mean loss of those steps; *pairs: pairs used. */
+int csrst_warmstart_train_iterates(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b,
+                                   int nrhs, int kmax, int inloop, float lr, float *loss, int *pairs);
 
 #ifdef __cplusplus
 }

@@ -141,3 +141,42 @@ def test_training_matches_cpu_trajectory(gpu):
     assert abs(gpu_l[0] - cpu_l[0]) < 2e-2 * cpu_l[0]
     assert cpu_l[-1] < 0.9 * cpu_l[0]
     assert gpu_l[-1] < 0.9 * gpu_l[0]
+
+
+def test_solve_modes_and_iterate_training(gpu):
+    """csrst_warmstart_solve: mode 0 is the multigrid of csrst_poisson_solve
+    (same V-cycles, same solution bit for bit); training on the solver's own
+    iterates lowers the loss; with the network's step before every V-cycle
+    (mode 1) or before the first (mode 2) the solve reaches the same tolerance
+    and the same potential."""
+    rng = np.random.default_rng(5)
+    H, W, h = 64, 128, 80e-6
+    cls = np.zeros((H, W), np.uint8)
+    cls[1:-1, 1:-1] = P.UNKNOWN
+
+    def sample():
+        rho = make_sample(rng, H, W, h)
+        rho[cls != P.UNKNOWN] = 0.0
+        return rho
+
+    u = WarmStart(H, W, max_batch=4)
+    rho = sample()
+    phi0, c0, r0 = P.solve(h, h, cls, rho, tol=1e-10, maxit=200)
+    phi_m0, cm0, rm0 = u.solve(h, h, cls, rho, tol=1e-10, maxit=200, mode=0)
+    assert cm0 == c0 and rm0 == r0
+    assert np.array_equal(phi_m0, phi0)
+    losses = []
+    for it in range(40):
+        loss, n = u.train_iterates(h, h, cls, np.stack([sample(), sample()]), kmax=8, inloop=it >= 20, lr=3e-4)
+        assert n > 0
+        losses.append(loss)
+    print("iterate-pair loss %.3f -> %.3f" % (np.mean(losses[:5]), np.mean(losses[-5:])))
+    assert np.mean(losses[-5:]) < 0.8 * np.mean(losses[:5])
+    for mode in (1, 2):
+        phi, c, r = u.solve(h, h, cls, rho, tol=1e-10, maxit=200, mode=mode)
+        print("mode %d: %d V-cycles (plain %d), residual %.1e" % (mode, c, c0, r))
+        assert r <= 1e-10
+        assert np.abs(phi - phi0).max() <= 1e-8 * np.abs(phi0).max()
+    with pytest.raises(ValueError):
+        u.solve(h, h, cls[:-1], rho)
+    u.close()
