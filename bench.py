@@ -159,14 +159,17 @@ def warmstart_density(rng, H, W, h):
 
 
 def warmstart_experiment(H=1024, W=2048, h=10e-6, train_seconds=300.0, test=1000, kmax=12, nrhs=2, seed=11,
-                         load=None, save=None):
+                         load=None, save=None, oneshot=100):
     """The learned start on 2048 x 1024 densities, network between V-cycles:
     (1) train on (residual, error) pairs of the multigrid's own iterates, the
     first half of the time on plain V-cycle iterates, the second on the
     iterates of the learned loop; (2) on `test` held-out densities solve to a
     relative residual of 1e-10 plainly from zero, with the network's step
-    before every V-cycle, and with it before the first only; report V-cycles,
-    final residuals and wall time per solve (host copies included).  load:
+    before every V-cycle, with the step over the network's direction and the
+    previous change, and over the previous change alone; on the first
+    `oneshot` of them also with the network before the first V-cycle only;
+    report V-cycles, final residuals and wall time per solve (host copies
+    included).  load:
     a weight file (numpy .npy) to start from; save: where to write the
     trained weights."""
     import numpy as np
@@ -195,17 +198,17 @@ def warmstart_experiment(H=1024, W=2048, h=10e-6, train_seconds=300.0, test=1000
     if save:
         np.save(save, u.weights())
     trng = np.random.default_rng(seed + 1000003)
-    res = {0: [], 1: [], 2: []}
+    res = {m: [] for m in range(5)}
     for i in range(test):
         b = warmstart_density(trng, H, W, h)
-        for mode in (0, 1, 2):
+        for mode in (0, 1, 3, 4, 2) if i < oneshot else (0, 1, 3, 4):
             t1 = time.perf_counter()
             _, c, r = u.solve(h, h, cls, b, tol=1e-10, maxit=100, mode=mode)
             res[mode].append((c, r, time.perf_counter() - t1))
-        if (i + 1) % 100 == 0:
-            print("warmstart: %d held-out densities, mean V-cycles plain %.2f, in-loop %.2f, one-shot %.2f" % (
-                i + 1, np.mean([x[0] for x in res[0]]), np.mean([x[0] for x in res[1]]),
-                np.mean([x[0] for x in res[2]])), file=sys.stderr, flush=True)
+        if (i + 1) % 50 == 0:
+            print("warmstart: %d held-out densities, mean V-cycles: plain %.2f, network %.2f, network + previous "
+                  "change %.2f, previous change %.2f" % (i + 1, *[np.mean([x[0] for x in res[m]]) for m in (0, 1, 3, 4)]),
+                  file=sys.stderr, flush=True)
     u.close()
 
     def summary(v):
@@ -217,10 +220,11 @@ def warmstart_experiment(H=1024, W=2048, h=10e-6, train_seconds=300.0, test=1000
     out = dict(grid="%dx%d" % (W, H), h_m=h, tol=1e-10, start=load or "He initialisation", train_seconds=ttrain,
                train_calls=calls, train_pairs=pairs, loss_first=float(np.mean(losses[:20])) if losses else None,
                loss_last=float(np.mean(losses[-20:])) if losses else None, test_densities=test,
-               plain=summary(res[0]), inloop=summary(res[1]), oneshot=summary(res[2]))
-    out["vcycle_ratio_inloop"] = out["plain"]["vcycles_mean"] / out["inloop"]["vcycles_mean"]
-    out["vcycle_ratio_oneshot"] = out["plain"]["vcycles_mean"] / out["oneshot"]["vcycles_mean"]
-    out["time_ratio_inloop"] = out["plain"]["ms_per_solve"] / out["inloop"]["ms_per_solve"]
+               plain=summary(res[0]), inloop=summary(res[1]), inloop_prev=summary(res[3]), prev_only=summary(res[4]),
+               oneshot=summary(res[2]), oneshot_densities=len(res[2]))
+    for k in ("inloop", "inloop_prev", "prev_only", "oneshot"):
+        out["vcycle_ratio_" + k] = out["plain"]["vcycles_mean"] / out[k]["vcycles_mean"]
+        out["time_ratio_" + k] = out["plain"]["ms_per_solve"] / out[k]["ms_per_solve"]
     return out
 
 

@@ -949,6 +949,7 @@ struct csrst_warmstart {
         size_t unknowns = 0;
         DBuf<uint8_t> dcls;
         DBuf<double> b, phi, pstar, part;
+        DBuf<double> pprev, rprev;     // the previous iterate and its residual (two-direction step)
         DBuf<bf16> pairIn;             // [maxBatch][H][W] r / rms(r)
         DBuf<float> pairTgt;           // [maxBatch][H][W] e / rms(e)
     } loop;
@@ -1257,6 +1258,72 @@ __global__ void __launch_bounds__(kB) k_err_sumsq(const double *pstar, const dou
     block_sum2(s, z, part);
 }
 
+// The two-direction step: partial sums over the unknowns of c.Ac, c.Ad, d.Ad,
+// c.r, d.r with d = phi - pprev (the previous iteration's change) and
+// A d = rprev - r; c may be null (d alone).  Five doubles per block.
+__global__ void __launch_bounds__(kB) k_step_dots5(pmg::Level L, const float *c, const double *pprev,
+                                                   const double *rprev, double *part)
+{
+    __shared__ double sh[5][kB];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const size_t n = (size_t)L.nx * L.ny;
+    for (size_t k = (size_t)blockIdx.x * kB + threadIdx.x; k < n; k += (size_t)gridDim.x * kB) {
+        if (!L.act[k]) continue;
+        const double r = L.res[k], d = L.phi[k] - pprev[k], ad = rprev[k] - r;
+        double ck = 0.0, ack = 0.0;
+        if (c) {
+            const int i = (int)(k % L.nx), j = (int)(k / L.nx);
+            ck = c[k];
+            double sx = 0.0, sy = 0.0;
+            if (i > 0 && L.act[k - 1]) sx += c[k - 1];
+            if (i + 1 < L.nx && L.act[k + 1]) sx += c[k + 1];
+            if (j > 0 && L.act[k - L.nx]) sy += c[k - L.nx];
+            if (j + 1 < L.ny && L.act[k + L.nx]) sy += c[k + L.nx];
+            ack = 2.0 * (L.cx + L.cy) * ck - L.cx * sx - L.cy * sy;
+        }
+        s[0] += ck * ack;
+        s[1] += ck * ad;
+        s[2] += d * ad;
+        s[3] += ck * r;
+        s[4] += d * r;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int o = kB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[5 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_sum5_final(double *part, int nb)
+{
+    if (threadIdx.x >= 5 || blockIdx.x != 0) return;
+    double a = 0.0;
+    for (int k = 0; k < nb; ++k) a += part[5 * k + threadIdx.x];
+    part[5 * nb + threadIdx.x] = a;
+}
+
+// phi += alpha c + beta (phi - pprev) on the unknowns; pprev, rprev take the
+// iterate and residual before the step
+__global__ void k_update2(pmg::Level L, const float *c, double alpha, double beta, double *pprev, double *rprev)
+{
+    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
+    if (k >= (size_t)L.nx * L.ny) return;
+    if (!L.act[k]) {
+        pprev[k] = 0.0;
+        rprev[k] = 0.0;
+        return;
+    }
+    const double ph = L.phi[k], d = ph - pprev[k];
+    pprev[k] = ph;
+    rprev[k] = L.res[k];
+    L.phi[k] = ph + (c ? alpha * (double)c[k] : 0.0) + beta * d;
+}
+
 __global__ void k_sum2_final(double *part, int nb)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -1313,7 +1380,8 @@ void loop_grid(csrst_warmstart *u, double hx, double hy, const uint8_t *cls)
     Lp.ready = false;
     Lp.cls.assign(cls, cls + n);
     Lp.dcls.alloc(n);
-    Lp.b.alloc(n); Lp.phi.alloc(n); Lp.pstar.alloc(n); Lp.part.alloc(2 * kRed + 2);
+    Lp.b.alloc(n); Lp.phi.alloc(n); Lp.pstar.alloc(n); Lp.part.alloc(5 * kRed + 5);
+    Lp.pprev.alloc(n); Lp.rprev.alloc(n);
     HU_CHECK(hipMemcpyAsync(Lp.dcls.p, cls, n, hipMemcpyHostToDevice, u->s));
     Lp.mg.setup(u->W, u->H, hx, hy, Lp.dcls.p, u->s);
     Lp.hx = hx; Lp.hy = hy; Lp.unknowns = unk;
@@ -1351,6 +1419,51 @@ void loop_correct(csrst_warmstart *u, double rn)
         k_axpy_act<<<nb(n), kB, 0, u->s>>>(L0, u->outf.p, cr / cac);
         HU_LAUNCH_CHECK();
     }
+}
+
+// The step over the network's direction (useNet) and the previous iteration's
+// change: the energy-norm minimiser over span{c, d}, a 2 x 2 system (one
+// direction when the other is absent or degenerate).  first: no previous
+// iterate yet (d = 0).
+void loop_step2(csrst_warmstart *u, double rn, bool useNet, bool first)
+{
+    auto &Lp = u->loop;
+    const pmg::Level &L0 = Lp.mg.level0();
+    const size_t n = (size_t)u->H * u->W;
+    const double rms = rn / std::sqrt((double)Lp.unknowns);
+    if (!(rms > 0)) return;
+    if (first) {
+        k_copy_d<<<nb(n), kB, 0, u->s>>>(Lp.pprev.p, L0.phi, n);
+        k_copy_d<<<nb(n), kB, 0, u->s>>>(Lp.rprev.p, L0.res, n);
+    }
+    const float *c = nullptr;
+    if (useNet) {
+        plan(u, 1);
+        k_res_in<<<nb(n), kB, 0, u->s>>>(L0.res, L0.act, u->ptr(u->A[0]), n, 1.0 / rms);
+        forward(u, 1);
+        c = u->outf.p;
+    }
+    k_step_dots5<<<kRed, kB, 0, u->s>>>(L0, c, Lp.pprev.p, Lp.rprev.p, Lp.part.p);
+    k_sum5_final<<<1, 64, 0, u->s>>>(Lp.part.p, kRed);
+    HU_LAUNCH_CHECK();
+    double h[5];
+    HU_CHECK(hipMemcpyAsync(h, Lp.part.p + 5 * kRed, sizeof h, hipMemcpyDeviceToHost, u->s));
+    HU_CHECK(hipStreamSynchronize(u->s));
+    const double cac = h[0], cad = h[1], dad = h[2], cr = h[3], dr = h[4];
+    double alpha = 0.0, beta = 0.0;
+    const bool hc = c && cac > 0, hd = dad > 0;
+    const double det = cac * dad - cad * cad;
+    if (hc && hd && det > 1e-12 * cac * dad) {
+        alpha = (cr * dad - cad * dr) / det;
+        beta = (cac * dr - cad * cr) / det;
+    } else if (hc) {
+        alpha = cr / cac;
+    } else if (hd) {
+        beta = dr / dad;
+    }
+    if (!std::isfinite(alpha) || !std::isfinite(beta)) alpha = beta = 0.0;
+    k_update2<<<nb(n), kB, 0, u->s>>>(L0, c, alpha, beta, Lp.pprev.p, Lp.rprev.p);
+    HU_LAUNCH_CHECK();
 }
 
 // One Adam step on the first B staged pairs; returns the loss before it.
@@ -1528,13 +1641,15 @@ int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float
 // Poisson solve on the network's H x W node grid (nx = W, ny = H) with the
 // multigrid of csrst_poisson_solve.  mode 0: plain V-cycles from phi; 1: the
 // network's step before every V-cycle; 2: before the first only (a one-shot
-// start).  phi: in the guess, out the solution; cycles and the final relative
+// start); 3: before every V-cycle the energy-norm minimiser over the
+// network's direction and the previous iteration's change; 4: over the
+// previous change alone (the same acceleration without the network).  phi: in the guess, out the solution; cycles and the final relative
 // residual as csrst_poisson_solve reports them.
 int csrst_warmstart_solve(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b, double *phi,
                           double tol, int maxit, int mode, int *cycles, double *resid)
 {
     try {
-        if (!u || !b || !phi || maxit < 0 || !(tol >= 0) || mode < 0 || mode > 2)
+        if (!u || !b || !phi || maxit < 0 || !(tol >= 0) || mode < 0 || mode > 4)
             throw Error(CSRST_EINVAL, "bad warm-start solve arguments");
         loop_grid(u, hx, hy, cls);
         auto &Lp = u->loop;
@@ -1547,6 +1662,7 @@ int csrst_warmstart_solve(csrst_warmstart *u, double hx, double hy, const uint8_
         int cyc = 0, flat = 0;
         while (r > target && cyc < maxit && flat < 3) {
             if (mode == 1 || (mode == 2 && cyc == 0)) loop_correct(u, r);
+            else if (mode >= 3) loop_step2(u, r, mode == 3, cyc == 0);
             Lp.mg.cycle();
             ++cyc;
             const double rn = Lp.mg.resnorm();

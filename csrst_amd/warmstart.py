@@ -88,7 +88,9 @@ class WarmStart:
     def solve(self, hx, hy, cls, b, phi0=None, tol=1e-10, maxit=200, mode=1):
         """Poisson solve on the network's grid (csrst_warmstart_solve): mode 0
         plain V-cycles, 1 the network's step before every V-cycle, 2 before the
-        first only.  Returns (phi, cycles, relative residual)."""
+        first only, 3 the step over the network's direction and the previous
+        change, 4 over the previous change alone.  Returns (phi, cycles,
+        relative residual)."""
         cls = self._grid(cls)
         b = np.ascontiguousarray(b, np.float64)
         if b.shape != cls.shape:

@@ -334,8 +334,11 @@ int csrst_warmstart_train_step(csrst_warmstart *u, const float *rho, const float
  * iterate moves by alpha c with alpha = (c . r_k) / (c . A c), the step that
  * minimises the energy norm of the error along c; a V-cycle follows.
  * mode 0: plain V-cycles from phi; 1: the network's step before every V-cycle;
- * 2: before the first V-cycle only.  b, phi: [H][W] doubles (host); phi in:
- * the guess, out: the solution; cycles and resid as csrst_poisson_solve. */
+ * 2: before the first V-cycle only; 3: before every V-cycle the energy-norm
+ * minimiser over span{c, phi_k - phi_k-1} (the network's direction and the
+ * previous iteration's change); 4: over the previous change alone.
+ * b, phi: [H][W] doubles (host); phi in: the guess, out: the solution;
+ * cycles and resid as csrst_poisson_solve. */
 int csrst_warmstart_solve(csrst_warmstart *u, double hx, double hy, const uint8_t *cls, const double *b,
                           double *phi, double tol, int maxit, int mode, int *cycles, double *resid);
 /* Training on the solver's own iterates: for each of the nrhs right-hand sides

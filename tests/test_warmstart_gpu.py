@@ -147,8 +147,9 @@ def test_solve_modes_and_iterate_training(gpu):
     """csrst_warmstart_solve: mode 0 is the multigrid of csrst_poisson_solve
     (same V-cycles, same solution bit for bit); training on the solver's own
     iterates lowers the loss; with the network's step before every V-cycle
-    (mode 1) or before the first (mode 2) the solve reaches the same tolerance
-    and the same potential."""
+    (mode 1), before the first (mode 2), with the previous change as a second
+    direction (mode 3) and the previous change alone (mode 4) the solve
+    reaches the same tolerance and the same potential."""
     rng = np.random.default_rng(5)
     H, W, h = 64, 128, 80e-6
     cls = np.zeros((H, W), np.uint8)
@@ -172,7 +173,7 @@ def test_solve_modes_and_iterate_training(gpu):
         losses.append(loss)
     print("iterate-pair loss %.3f -> %.3f" % (np.mean(losses[:5]), np.mean(losses[-5:])))
     assert np.mean(losses[-5:]) < 0.8 * np.mean(losses[:5])
-    for mode in (1, 2):
+    for mode in (1, 2, 3, 4):
         phi, c, r = u.solve(h, h, cls, rho, tol=1e-10, maxit=200, mode=mode)
         print("mode %d: %d V-cycles (plain %d), residual %.1e" % (mode, c, c0, r))
         assert r <= 1e-10
