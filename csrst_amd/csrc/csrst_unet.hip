@@ -13,7 +13,9 @@
 // channel).  The same kernel, with transposed weights and mirrored taps, gives
 // the input gradient.  The one-channel input and output layers are load-bound
 // and run on plain vector kernels.  The weight gradient is a GEMM over pixels
-// split across workgroups with FP32 atomics (16x16x16 MFMA).
+// on the same MFMA, split across workgroups with FP32 atomics (k_wgrad32,
+// k_wgrad_one).  k_conv and k_conv_wgrad (16x16x16) remain for channel counts
+// the 32-channel kernels do not take.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -573,137 +575,263 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(const bf16 *dy, const bf16 *x
     }
 }
 
-// Weight gradient for Cin and Cout multiples of 16: per step of 16 pixels
-// the dY tile (16 pixels x 16 output channels) and the four shifted X tiles
-// (16 pixels x 16 input channels each; a tile of 16 n = tap Cin + ci lies in
-// one tap) are read with 8-byte loads (4 channels of one pixel per lane),
-// staged in LDS and read back in the MFMA operand layout.  Same grid and
-// results as k_conv_wgrad up to the order of the FP32 sums.
-__global__ void __launch_bounds__(64) k_conv_wgrad16(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
-                                                    int Cin, int Cout, int chunk)
+// Weight gradient for Cin and Cout multiples of 32 on v_mfma_f32_16x16x32_bf16:
+// dW[co][tap][ci] = sum_p dY[p][co] X[p + s(tap)][ci], K = pixels.  A
+// workgroup (4 waves) owns a block of 16 CF WCO output channels x 64 / WCO
+// input channels, all 9 taps, and sums over the 8 x 32 pixel tiles
+// blockIdx / nBlk, + ksplit, ...; wave w takes output-channel fragments
+// (w % WCO) CF .. + CF - 1 and input-channel fragment w / WCO.  Per tile dY
+// and the input with its halo are staged in LDS transposed (pixels fastest,
+// 4 pixels per 8-byte write), so a lane's 8 consecutive pixels of a row are
+// one 16-byte read.  Per K step (one tile row) and tap row dy, two aligned
+// reads of the input row give the three dx shifts by funnel shifts of the
+// lane's registers.  Row pitches are padded so that 16 lanes' 16-byte reads
+// fall on distinct banks.  One atomic per weight and workgroup at the end.
+constexpr int kWgTH = 8, kWgTW = 32;          // pixel tile
+constexpr int kDyPitch = kWgTH * kWgTW + 8;   // bf16 per output channel in LDS
+constexpr int kXCols = 40;                    // input columns x0 - 1 .. x0 + 38
+constexpr int kXPitch = (kWgTH + 2) * kXCols + 8;   // bf16 per input channel in LDS
+
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf8 as_bf8(u4v v) { return __builtin_bit_cast(bf8, v); }
+
+template <int CF, int WCO>
+__global__ void __launch_bounds__(kB, 2) k_wgrad32(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
+                                                   int Cin, int Cout, int ksplit)
 {
-    __shared__ bf16 sA[16][16 + 4];
-    __shared__ bf16 sB[4][16][16 + 4];
-    const int lane = threadIdx.x, li = lane & 15, kg = lane >> 4;
-    const int P = B * H * W, N = 9 * Cin;
-    const int co0 = blockIdx.y * 16;
-    const int n0 = blockIdx.x * 64;
-    const int k0 = blockIdx.z * chunk, k1 = min(P, k0 + chunk);
-    const int lp = lane >> 2, lc = (lane & 3) * 4;   // loader: pixel and 4 channels
-    f4 acc[4];
+    constexpr int CO_BLK = 16 * CF * WCO, CI_BLK = 16 * (4 / WCO);
+    __shared__ __attribute__((aligned(16))) bf16 sD[CO_BLK * kDyPitch];
+    __shared__ __attribute__((aligned(16))) bf16 sXt[CI_BLK * kXPitch];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kg = lane >> 4;
+    const int nCoB = Cout / CO_BLK, nCiB = Cin / CI_BLK, nBlk = nCoB * nCiB;
+    const int blk = blockIdx.x % nBlk, kz = blockIdx.x / nBlk;
+    const int co0 = (blk % nCoB) * CO_BLK, ci0 = (blk / nCoB) * CI_BLK;
+    const int cf0 = (wave % WCO) * CF, nf = wave / WCO;
+    const int tilesX = (W + kWgTW - 1) / kWgTW, tilesY = (H + kWgTH - 1) / kWgTH, nTiles = B * tilesX * tilesY;
+    f4 acc[CF][9];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    int tapt[4], ci0[4];
-    bool tv[4];
+    for (int m = 0; m < CF; ++m)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int n = n0 + 16 * t;
-        tv[t] = n < N;
-        tapt[t] = tv[t] ? n / Cin : 0;
-        ci0[t] = tv[t] ? n % Cin : 0;
-    }
-    for (int kb = k0; kb < k1; kb += 16) {
-        const int p = kb + lp;
-        uint2 va = make_uint2(0u, 0u);
-        uint2 vb[4] = {make_uint2(0u, 0u), make_uint2(0u, 0u), make_uint2(0u, 0u), make_uint2(0u, 0u)};
-        if (p < k1) {
-            va = *reinterpret_cast<const uint2 *>(dy + (size_t)p * Cout + co0 + lc);
-            const int xx = p % W, yy = (p / W) % H, bb = p / (W * H);
+        for (int t = 0; t < 9; ++t) acc[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int tile = kz; tile < nTiles; tile += ksplit) {
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int x0 = tx * kWgTW, y0 = (rest % tilesY) * kWgTH, bimg = rest / tilesY;
+        __syncthreads();
+        // dY: units (row, 4-pixel group, 8-channel group)
+        for (int u = tid; u < kWgTH * (kWgTW / 4) * (CO_BLK / 8); u += kB) {
+            const int cg = u % (CO_BLK / 8), r2 = u / (CO_BLK / 8);
+            const int x4 = r2 % (kWgTW / 4), r = r2 / (kWgTW / 4);
+            const int yy = y0 + r;
+            uint4 v[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int ys = yy + tapt[t] / 3 - 1, xs = xx + tapt[t] % 3 - 1;
-                if (tv[t] && ys >= 0 && ys < H && xs >= 0 && xs < W)
-                    vb[t] = *reinterpret_cast<const uint2 *>(x + ((size_t)(bb * H + ys) * W + xs) * Cin + ci0[t] + lc);
+            for (int q = 0; q < 4; ++q) {
+                const int xx = x0 + 4 * x4 + q;
+                v[q] = (yy < H && xx < W)
+                           ? *reinterpret_cast<const uint4 *>(dy + (((size_t)bimg * H + yy) * W + xx) * Cout + co0 + 8 * cg)
+                           : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                auto h = [&](const uint4 &w) -> uint32_t {
+                    const uint32_t d = e < 2 ? w.x : e < 4 ? w.y : e < 6 ? w.z : w.w;
+                    return (e & 1) ? (d >> 16) : (d & 0xFFFFu);
+                };
+                const uint2 o = make_uint2(h(v[0]) | (h(v[1]) << 16), h(v[2]) | (h(v[3]) << 16));
+                *reinterpret_cast<uint2 *>(&sD[(8 * cg + e) * kDyPitch + r * kWgTW + 4 * x4]) = o;
+            }
+        }
+        // input with its halo: units (row, 4-column group, 8-channel group)
+        for (int u = tid; u < (kWgTH + 2) * (kXCols / 4) * (CI_BLK / 8); u += kB) {
+            const int ig = u % (CI_BLK / 8), r2 = u / (CI_BLK / 8);
+            const int c4 = r2 % (kXCols / 4), r = r2 / (kXCols / 4);
+            const int ys = y0 + r - 1;
+            uint4 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int xs = x0 - 1 + 4 * c4 + q;
+                v[q] = (ys >= 0 && ys < H && xs >= 0 && xs < W)
+                           ? *reinterpret_cast<const uint4 *>(x + (((size_t)bimg * H + ys) * W + xs) * Cin + ci0 + 8 * ig)
+                           : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                auto h = [&](const uint4 &w) -> uint32_t {
+                    const uint32_t d = e < 2 ? w.x : e < 4 ? w.y : e < 6 ? w.z : w.w;
+                    return (e & 1) ? (d >> 16) : (d & 0xFFFFu);
+                };
+                const uint2 o = make_uint2(h(v[0]) | (h(v[1]) << 16), h(v[2]) | (h(v[3]) << 16));
+                *reinterpret_cast<uint2 *>(&sXt[(8 * ig + e) * kXPitch + r * kXCols + 4 * c4]) = o;
             }
         }
         __syncthreads();
-        *reinterpret_cast<uint2 *>(&sA[lp][lc]) = va;
+#pragma unroll 1
+        for (int r = 0; r < kWgTH; ++r) {
+            bf8 av[CF];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) *reinterpret_cast<uint2 *>(&sB[t][lp][lc]) = vb[t];
-        __syncthreads();
-        sh4 av;
+            for (int m = 0; m < CF; ++m)
+                av[m] = *reinterpret_cast<const bf8 *>(&sD[(16 * (cf0 + m) + li) * kDyPitch + r * kWgTW + 8 * kg]);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) av[r] = (short)sA[4 * kg + r][li];
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                const bf16 *row = &sXt[(16 * nf + li) * kXPitch + (r + dyi) * kXCols + 8 * kg];
+                const u4v f0 = *reinterpret_cast<const u4v *>(row), f1 = *reinterpret_cast<const u4v *>(row + 8);
+                u4v s1, s2;   // shifted by one and by two columns
+                s1[0] = __builtin_amdgcn_alignbit(f0[1], f0[0], 16);
+                s1[1] = __builtin_amdgcn_alignbit(f0[2], f0[1], 16);
+                s1[2] = __builtin_amdgcn_alignbit(f0[3], f0[2], 16);
+                s1[3] = __builtin_amdgcn_alignbit(f1[0], f0[3], 16);
+                s2[0] = f0[1]; s2[1] = f0[2]; s2[2] = f0[3]; s2[3] = f1[0];
+                const bf8 bv[3] = {as_bf8(f0), as_bf8(s1), as_bf8(s2)};
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            sh4 bv;
+                for (int dxi = 0; dxi < 3; ++dxi)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) bv[r] = (short)sB[t][4 * kg + r][li];
-            acc[t] = mfma(av, bv, acc[t]);
+                    for (int m = 0; m < CF; ++m)
+                        acc[m][3 * dyi + dxi] = mfma32(av[m], bv[dxi], acc[m][3 * dyi + dxi]);
+            }
         }
     }
+    // D[4 kg + rr][li]: output channel co0 + 16 (cf0 + m) + 4 kg + rr, input channel ci0 + 16 nf + li
+    const int ci = ci0 + 16 * nf + li;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        if (!tv[t]) continue;
-        const int n = n0 + 16 * t + li;
+    for (int m = 0; m < CF; ++m)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int c = co0 + 4 * kg + r;
-            atomicAdd(&dw[(size_t)c * N + n], acc[t][r]);
-        }
-    }
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int co = co0 + 16 * (cf0 + m) + 4 * kg + rr;
+                atomicAdd(&dw[((size_t)co * 9 + t) * Cin + ci], acc[m][t][rr]);
+            }
 }
 
-// Weight gradient of the one-channel input and output layers (Cin = 1 or
-// Cout = 1, Cout 9 Cin <= 512 weights): thread o of a block owns weights o and
-// o + 256 (dW[co][tap][ci] in the order of the weight layout) and sums them over
-// the 8 x 32 pixel tiles blockIdx.x, + gridDim.x, ...; per tile dY and the input
-// with its halo are staged in LDS as floats.  One atomic per weight and block.
-constexpr int kSmallC = 32;      // widest channel count of the staged operand
-
-__global__ void __launch_bounds__(kB) k_wgrad_small(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W,
-                                                    int Cin, int Cout)
+// Weight gradient of the one-channel layers on the same MFMA path: OUT1 is the
+// output layer (Cin = 32, Cout = 1), otherwise the input layer (Cin = 1,
+// Cout = 32).  The one channel fills row (or column) 0 of its 16-wide MFMA
+// operand and the other lanes hold zeros.  Wave w takes 16-channel fragment
+// w & 1 of the 32-channel side and tile rows (w >> 1) + 2 j.  Staging and
+// shifts as in k_wgrad32; pixel tiles blockIdx.x, + gridDim.x, ...
+template <bool OUT1>
+__global__ void __launch_bounds__(kB) k_wgrad_one(const bf16 *dy, const bf16 *x, float *dw, int B, int H, int W)
 {
-    __shared__ float sD[kTH * kTW * kSmallC];
-    __shared__ float sXs[kIH * kIW * kSmallC];
-    const int tilesX = (W + kTW - 1) / kTW, tilesY = (H + kTH - 1) / kTH, nTiles = B * tilesX * tilesY;
-    const int N = Cout * 9 * Cin;
-    int co[2], off[2];
-    bool own[2];
+    constexpr int NCO = OUT1 ? 1 : 32, NCI = OUT1 ? 32 : 1;
+    __shared__ __attribute__((aligned(16))) bf16 sD[NCO * kDyPitch];
+    __shared__ __attribute__((aligned(16))) bf16 sXt[NCI * kXPitch];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kg = lane >> 4, fr = wave & 1;
+    const int tilesX = (W + kWgTW - 1) / kWgTW, tilesY = (H + kWgTH - 1) / kWgTH, nTiles = B * tilesX * tilesY;
+    f4 acc[9];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int o = threadIdx.x + kB * j;
-        own[j] = o < N;
-        const int oo = own[j] ? o : 0;
-        co[j] = oo / (9 * Cin);
-        const int tap = (oo / Cin) % 9, ci = oo % Cin;
-        off[j] = ((tap / 3) * kIW + tap % 3) * Cin + ci;   // input offset of the tap from the pixel's top-left halo
-    }
-    float acc[2] = {0.f, 0.f};
-    for (int t = blockIdx.x; t < nTiles; t += gridDim.x) {
-        int tt = t;
-        const int tx = tt % tilesX;
-        tt /= tilesX;
-        const int ty = tt % tilesY;
-        const int b = tt / tilesY;
-        const int x0 = tx * kTW, y0 = ty * kTH;
+    for (int t = 0; t < 9; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const u4v zero = {0u, 0u, 0u, 0u};
+    for (int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int x0 = tx * kWgTW, y0 = (rest % tilesY) * kWgTH, bimg = rest / tilesY;
         __syncthreads();
-        for (int k = threadIdx.x; k < kTH * kTW * Cout; k += kB) {
-            const int px = k / Cout, c = k - px * Cout;
-            const int yy = y0 + px / kTW, xx = x0 + px % kTW;
-            sD[k] = (yy < H && xx < W) ? bf2f(dy[(((size_t)b * H + yy) * W + xx) * Cout + c]) : 0.f;
-        }
-        for (int k = threadIdx.x; k < kIH * kIW * Cin; k += kB) {
-            const int px = k / Cin, c = k - px * Cin;
-            const int ys = y0 + px / kIW - 1, xs = x0 + px % kIW - 1;
-            sXs[k] = (ys >= 0 && ys < H && xs >= 0 && xs < W) ? bf2f(x[(((size_t)b * H + ys) * W + xs) * Cin + c]) : 0.f;
-        }
-        __syncthreads();
+        // dY: units (row, 4-pixel group, 8-channel group); one channel: 4 scalars
+        for (int u = tid; u < kWgTH * (kWgTW / 4) * ((NCO + 7) / 8); u += kB) {
+            const int cg = u % ((NCO + 7) / 8), r2 = u / ((NCO + 7) / 8);
+            const int x4 = r2 % (kWgTW / 4), r = r2 / (kWgTW / 4);
+            const int yy = y0 + r;
+            if (NCO == 1) {
+                uint32_t h[4];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (!own[j]) continue;
-            float s = 0.f;
+                for (int q = 0; q < 4; ++q) {
+                    const int xx = x0 + 4 * x4 + q;
+                    h[q] = (yy < H && xx < W) ? (uint32_t)dy[((size_t)bimg * H + yy) * W + xx] : 0u;
+                }
+                *reinterpret_cast<uint2 *>(&sD[r * kWgTW + 4 * x4]) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            } else {
+                uint4 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xx = x0 + 4 * x4 + q;
+                    v[q] = (yy < H && xx < W)
+                               ? *reinterpret_cast<const uint4 *>(dy + (((size_t)bimg * H + yy) * W + xx) * NCO + 8 * cg)
+                               : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    auto h = [&](const uint4 &w) -> uint32_t {
+                        const uint32_t d = e < 2 ? w.x : e < 4 ? w.y : e < 6 ? w.z : w.w;
+                        return (e & 1) ? (d >> 16) : (d & 0xFFFFu);
+                    };
+                    *reinterpret_cast<uint2 *>(&sD[(8 * cg + e) * kDyPitch + r * kWgTW + 4 * x4]) =
+                        make_uint2(h(v[0]) | (h(v[1]) << 16), h(v[2]) | (h(v[3]) << 16));
+                }
+            }
+        }
+        // input with its halo: units (row, 4-column group, 8-channel group)
+        for (int u = tid; u < (kWgTH + 2) * (kXCols / 4) * ((NCI + 7) / 8); u += kB) {
+            const int ig = u % ((NCI + 7) / 8), r2 = u / ((NCI + 7) / 8);
+            const int c4 = r2 % (kXCols / 4), r = r2 / (kXCols / 4);
+            const int ys = y0 + r - 1;
+            if (NCI == 1) {
+                uint32_t h[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xs = x0 - 1 + 4 * c4 + q;
+                    h[q] = (ys >= 0 && ys < H && xs >= 0 && xs < W) ? (uint32_t)x[((size_t)bimg * H + ys) * W + xs] : 0u;
+                }
+                *reinterpret_cast<uint2 *>(&sXt[r * kXCols + 4 * c4]) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            } else {
+                uint4 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xs = x0 - 1 + 4 * c4 + q;
+                    v[q] = (ys >= 0 && ys < H && xs >= 0 && xs < W)
+                               ? *reinterpret_cast<const uint4 *>(x + (((size_t)bimg * H + ys) * W + xs) * NCI + 8 * ig)
+                               : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    auto h = [&](const uint4 &w) -> uint32_t {
+                        const uint32_t d = e < 2 ? w.x : e < 4 ? w.y : e < 6 ? w.z : w.w;
+                        return (e & 1) ? (d >> 16) : (d & 0xFFFFu);
+                    };
+                    *reinterpret_cast<uint2 *>(&sXt[(8 * ig + e) * kXPitch + r * kXCols + 4 * c4]) =
+                        make_uint2(h(v[0]) | (h(v[1]) << 16), h(v[2]) | (h(v[3]) << 16));
+                }
+            }
+        }
+        __syncthreads();
+        // the one-channel side's operand: lane li = 0 only
+        const bool one = li == 0;
 #pragma unroll 1
-            for (int py = 0; py < kTH; ++py)
-#pragma unroll 8
-                for (int px = 0; px < kTW; ++px)
-                    s += sD[(py * kTW + px) * Cout + co[j]] * sXs[(py * kIW + px) * Cin + off[j]];
-            acc[j] += s;
+        for (int j = 0; j < kWgTH / 2; ++j) {
+            const int r = (wave >> 1) + 2 * j;
+            bf8 av;
+            if (OUT1) av = one ? *reinterpret_cast<const bf8 *>(&sD[r * kWgTW + 8 * kg]) : as_bf8(zero);
+            else av = *reinterpret_cast<const bf8 *>(&sD[(16 * fr + li) * kDyPitch + r * kWgTW + 8 * kg]);
+#pragma unroll
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                u4v f0 = zero, f1 = zero;
+                if (!OUT1 ? one : true) {
+                    const bf16 *row = &sXt[(OUT1 ? (16 * fr + li) * kXPitch : 0) + (r + dyi) * kXCols + 8 * kg];
+                    f0 = *reinterpret_cast<const u4v *>(row);
+                    f1 = *reinterpret_cast<const u4v *>(row + 8);
+                }
+                u4v s1, s2;
+                s1[0] = __builtin_amdgcn_alignbit(f0[1], f0[0], 16);
+                s1[1] = __builtin_amdgcn_alignbit(f0[2], f0[1], 16);
+                s1[2] = __builtin_amdgcn_alignbit(f0[3], f0[2], 16);
+                s1[3] = __builtin_amdgcn_alignbit(f1[0], f0[3], 16);
+                s2[0] = f0[1]; s2[1] = f0[2]; s2[2] = f0[3]; s2[3] = f1[0];
+                acc[3 * dyi + 0] = mfma32(av, as_bf8(f0), acc[3 * dyi + 0]);
+                acc[3 * dyi + 1] = mfma32(av, as_bf8(s1), acc[3 * dyi + 1]);
+                acc[3 * dyi + 2] = mfma32(av, as_bf8(s2), acc[3 * dyi + 2]);
+            }
         }
     }
+    // D[4 kg + rr][li]: row = output channel (OUT1: only row 0), column = input channel (else: only column 0)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-        if (own[j]) atomicAdd(&dw[threadIdx.x + kB * j], acc[j]);
+    for (int t = 0; t < 9; ++t) {
+        if (OUT1) {
+            if (kg == 0) atomicAdd(&dw[t * 32 + 16 * fr + li], acc[t][0]);
+        } else if (li == 0) {
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) atomicAdd(&dw[(16 * fr + 4 * kg + rr) * 9 + t], acc[t][rr]);
+        }
+    }
 }
 
 // Elementwise kernels take 8 channels (one 16-byte vector) per thread: every
@@ -861,16 +989,18 @@ __global__ void k_to_bf(const float *x, bf16 *y, size_t n, float scale)
 }
 
 // MSE loss of the fp32 output against the target; gradient 2 (o - t) / n in bf16
+// (grid-stride over at most kMseBlocks blocks: one atomic per block)
+constexpr int kMseBlocks = 1024;
 __global__ void k_mse(const float *o, const float *t, bf16 *g, float *loss, size_t n)
 {
     __shared__ float sh[kB];
-    const size_t k = (size_t)blockIdx.x * kB + threadIdx.x;
-    float d = 0.f;
-    if (k < n) {
-        d = o[k] - t[k];
+    float acc = 0.f;
+    for (size_t k = (size_t)blockIdx.x * kB + threadIdx.x; k < n; k += (size_t)gridDim.x * kB) {
+        const float d = o[k] - t[k];
         g[k] = f2bf(2.f * d / (float)n);
+        acc += d * d;
     }
-    sh[threadIdx.x] = d * d;
+    sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = kB / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
@@ -1006,13 +1136,21 @@ void conv_wgrad(csrst_warmstart *u, int li, const bf16 *dpre, const bf16 *x, int
     const int P = B * H * W;
     const int chunk = 1024;
     dim3 grid((9 * l.cin + 63) / 64, pad16(l.cout) / 16, (P + chunk - 1) / chunk);
-    const bool small = (l.cin == 1 && l.cout <= kSmallC) || (l.cout == 1 && l.cin <= kSmallC);
-    if (small) {
-        const int tiles = B * ((H + kTH - 1) / kTH) * ((W + kTW - 1) / kTW);
-        k_wgrad_small<<<std::min(tiles, 1024), kB, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout);
-    } else if (l.cin % 16 == 0 && l.cout % 16 == 0)
-        k_conv_wgrad16<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
-    else
+    if (l.cin % 32 == 0 && l.cout % 32 == 0) {
+        const int tiles = B * ((H + kWgTH - 1) / kWgTH) * ((W + kWgTW - 1) / kWgTW);
+        auto go = [&](auto kern, int coBlk, int ciBlk) {
+            const int nBlk = (l.cout / coBlk) * (l.cin / ciBlk);
+            const int ksplit = std::max(1, std::min(tiles, 1024 / nBlk));
+            kern<<<nBlk * ksplit, kB, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, ksplit);
+        };
+        if (l.cin % 64 == 0) go(k_wgrad32<2, 1>, 32, 64);
+        else if (l.cout % 64 == 0) go(k_wgrad32<2, 2>, 64, 32);
+        else go(k_wgrad32<1, 2>, 32, 32);
+    } else if ((l.cin == 1 && l.cout == 32) || (l.cin == 32 && l.cout == 1)) {
+        const int tiles = B * ((H + kWgTH - 1) / kWgTH) * ((W + kWgTW - 1) / kWgTW);
+        if (l.cout == 1) k_wgrad_one<true><<<std::min(tiles, 1024), kB, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W);
+        else k_wgrad_one<false><<<std::min(tiles, 1024), kB, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W);
+    } else
         k_conv_wgrad<<<grid, 64, 0, u->s>>>(dpre, x, u->g.p + l.woff, B, H, W, l.cin, l.cout, chunk);
     if (l.cout % 8 == 0 && kB % (l.cout / 8) == 0) {
         const int rows = kB / (l.cout / 8);
@@ -1476,7 +1614,7 @@ float loop_train(csrst_warmstart *u, int B, float lr)
     HU_CHECK(hipMemcpyAsync(u->tgt.p, Lp.pairTgt.p, np * sizeof(float), hipMemcpyDeviceToDevice, u->s));
     forward(u, B);
     HU_CHECK(hipMemsetAsync(u->lossBuf.p, 0, sizeof(float), u->s));
-    k_mse<<<nb(np), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
+    k_mse<<<std::min(nb(np), kMseBlocks), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
     backward(u, B);
     ++u->steps;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -1607,7 +1745,7 @@ int csrst_warmstart_gradients(csrst_warmstart *u, const float *rho, const float 
         HU_CHECK(hipMemcpyAsync(u->tgt.p, phi, np * sizeof(float), hipMemcpyHostToDevice, u->s));
         forward(u, B);
         HU_CHECK(hipMemsetAsync(u->lossBuf.p, 0, sizeof(float), u->s));
-        k_mse<<<nb(np), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
+        k_mse<<<std::min(nb(np), kMseBlocks), kB, 0, u->s>>>(u->outf.p, u->tgt.p, u->ptr(u->Gd[21]), u->lossBuf.p, np);
         backward(u, B);
         if (loss) HU_CHECK(hipMemcpyAsync(loss, u->lossBuf.p, sizeof(float), hipMemcpyDeviceToHost, u->s));
         if (grads) HU_CHECK(hipMemcpyAsync(grads, u->g.p, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->s));

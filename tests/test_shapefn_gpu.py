@@ -146,3 +146,25 @@ def test_caller_geometry_and_stream(gpu):
     with pytest.raises(CsrstError):
         g3.set_boundary_cells(bad, data)      # a cell listed twice
     stream.close()
+
+
+def test_empty_bunch_and_one_node_patch(gpu):
+    """No particles: the deposits and the charge density are zero on the
+    patch, a scatter leaves the field as it was, a gather returns nothing.
+    The smallest patch that holds an S0 (CIC) charge, the 2 x 2 vertices of
+    its cell, gets the oracle's values."""
+    org, dx, dt = (0.0, 0.0), (0.7, 1.3), 0.37
+    none = np.zeros((0, 2))
+    for order in (0, 1, 2):
+        for comp in (0, 1):
+            J = SF.getCurrentDensity(order, comp, (0, 0, 7, 5), org, dx, dt, none, none, np.zeros(0))
+            assert J.shape == (5, 7) and not J.any()
+        assert not SF.getChargeDensity(order, (0, 0, 7, 5), org, dx, none, np.zeros(0)).any()
+        field = np.arange(35.0).reshape(5, 7)
+        assert same(SF.scatter(order, "cell", (5, 7), org, dx, none, np.zeros(0), field=field), field)
+        assert SF.gather(order, "Ex", field, org, dx, none).shape == (0,)
+    R, q = np.array([[3.2 * dx[0], 2.1 * dx[1]]]), np.array([2.0])
+    rho = SF.getChargeDensity(0, (3, 2, 2, 2), org, dx, R, q)
+    assert rho.shape == (2, 2) and same(rho, O.charge_density(0, (3, 2, 2, 2), org, dx, R, q))
+    with pytest.raises(CsrstError):
+        SF.getChargeDensity(0, (3, 2, 1, 1), org, dx, R, q)      # the stencil leaves a 1 x 1 patch
