@@ -297,8 +297,8 @@ __global__ void __launch_bounds__(kPB, 1) k_conv32p(ConvArgs a)
             _Pragma("unroll") for (int j = 0; j < kWPer; ++j)                                                        \
             {                                                                                                        \
                 const int s = tid + kPB * j;                                                                         \
-                const int seg = s & 3, r = s >> 2;                                                                   \
-                const int col = r / 9, tap = r - 9 * col;                                                            \
+                const int seg = s & 3, r = s >> 2;              /* LDS row r = tap * 16 MF + col */                \
+                const int tap = r / (16 * MF), col = r - tap * (16 * MF);                                            \
                 rw[j] = s < kWSeg ? *reinterpret_cast<const uint4 *>(a.w + ((size_t)(co0 + col) * 9 + tap) * Cin +  \
                                                                      c0 + 8 * seg)                                   \
                                   : make_uint4(0u, 0u, 0u, 0u);                                                      \
@@ -320,9 +320,8 @@ __global__ void __launch_bounds__(kPB, 1) k_conv32p(ConvArgs a)
             _Pragma("unroll") for (int j = 0; j < kWPer; ++j)                                                        \
             {                                                                                                        \
                 const int s = tid + kPB * j;                                                                         \
-                const int seg = s & 3, r = s >> 2;                                                                   \
-                const int col = r / 9, tap = r - 9 * col;                                                            \
-                if (s < kWSeg) *reinterpret_cast<uint4 *>(&sW[(tap * 16 * MF + col) * kPS + 8 * seg]) = rw[j];       \
+                const int seg = s & 3;                                                                               \
+                if (s < kWSeg) *reinterpret_cast<uint4 *>(&sW[(s >> 2) * kPS + 8 * seg]) = rw[j];                     \
             }                                                                                                        \
         __syncthreads();                                                                                             \
         if ((item) + DEPTH < nItems) CONV32P_FETCH((item) + DEPTH, !wOnce, rx, rw)                                   \
