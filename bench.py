@@ -228,6 +228,35 @@ def warmstart_experiment(H=1024, W=2048, h=10e-6, train_seconds=300.0, test=1000
     return out
 
 
+def poisson_bench(H=1024, W=2048, h=10e-6, n=20, seed=5):
+    """The FP64 multigrid on the c4 grid (2048 x 1024 nodes, grounded chamber)
+    and c4 densities: V-cycles to a relative residual of 1e-10 and device time
+    per solve and per V-cycle, plain and with the previous-change step
+    (csrst_warmstart_solve modes 0 and 4; the hierarchy stays on the device).
+    Wall times include the solve's host copies of b and phi."""
+    import numpy as np
+    from csrst_amd import poisson as PS
+    from csrst_amd.warmstart import WarmStart
+    cls = np.zeros((H, W), np.uint8)
+    cls[1:-1, 1:-1] = PS.UNKNOWN
+    u = WarmStart(H, W, max_batch=1)
+    rng = np.random.default_rng(seed)
+    dens = [warmstart_density(rng, H, W, h) for _ in range(n + 1)]
+    out = {}
+    for mode, name in ((0, "plain"), (4, "previous_change")):
+        u.solve(h, h, cls, dens[0], tol=1e-10, maxit=100, mode=mode)      # warm-up: hierarchy, code
+        cyc, t = [], []
+        for b in dens[1:]:
+            t0 = time.perf_counter()
+            _, c, r = u.solve(h, h, cls, b, tol=1e-10, maxit=100, mode=mode)
+            t.append(time.perf_counter() - t0)
+            cyc.append(c)
+        out[name] = dict(vcycles_mean=float(np.mean(cyc)), ms_per_solve=1e3 * float(np.mean(t)),
+                         ms_per_vcycle=1e3 * float(np.sum(t)) / float(np.sum(cyc)))
+    u.close()
+    return dict(grid="%dx%d" % (W, H), h_m=h, densities=n, tol=1e-10, **out)
+
+
 def _oracle_worker(seconds):
     """Child process of cpu_baseline's all-cores leg: one sample, as JSON."""
     print(json.dumps(_oracle_sample(seconds)), flush=True)
@@ -326,6 +355,8 @@ def main():
     ap.add_argument("--stat", type=int, default=20, help="wake sampling period (steps)")
     ap.add_argument("--warmstart", action="store_true",
                     help="only time the warm-start U-Net inference on 2048x1024 densities and print it")
+    ap.add_argument("--poisson", action="store_true",
+                    help="only time the FP64 multigrid on 2048x1024 c4 densities and print it")
     ap.add_argument("--warmstart-experiment", action="store_true",
                     help="only run the learned-start experiment (network between V-cycles, 2048x1024) and print it")
     ap.add_argument("--ws-train-seconds", type=float, default=300.0)
@@ -356,6 +387,9 @@ def main():
         return
     if a.warmstart:
         print(json.dumps(warmstart_bench()), flush=True)
+        return
+    if a.poisson:
+        print(json.dumps(poisson_bench()), flush=True)
         return
     if a.warmstart_experiment:
         gw, gh = (int(v) for v in a.ws_grid.split("x"))

@@ -148,7 +148,12 @@ __global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
     extern __shared__ double sm[];
     const int ny = L.ny;
     double *A = sm, *Cc = sm + ny, *D = sm + 2 * ny;
-    const int i = 2 * blockIdx.x + color;
+    // Consecutive workgroup ids run on the 8 XCDs in turn.  Each XCD gets a
+    // contiguous range of columns, so the columns that share a cache line
+    // (16 doubles of a row) are read through one L2.
+    int bid = blockIdx.x;
+    if (gridDim.x >= 64 && (gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    const int i = 2 * bid + color;
     if (i >= L.nx || !L.colact[i]) return;           // uniform over the workgroup
     const double diag = 2.0 * (L.cx + L.cy);
     const double inv = 1.0 / diag;
