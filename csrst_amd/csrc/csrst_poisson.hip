@@ -18,7 +18,6 @@ namespace pmg {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPcrThreads = 256;
 constexpr int kRedBlocks = 256;
 
 int nblk(size_t n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -142,8 +141,8 @@ __global__ void k_sum_final(double *partial, int nb)
 // with the x neighbours taken from the current iterate.  Each reduction step
 // eliminates the couplings at distance s using rows j -/+ s; after
 // ceil(log2 ny) steps every row is decoupled and phi_j = d_j.
-template <int R>
-__global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_line_pcr(Level L, int color)
 {
     extern __shared__ double sm[];
     const int ny = L.ny;
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
     const double inv = 1.0 / diag;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = threadIdx.x + r * kPcrThreads;
+        const int j = threadIdx.x + r * NT;
         if (j >= ny) break;
         const size_t k = (size_t)j * L.nx + i;
         double a = 0.0, c = 0.0, d = 0.0;
@@ -178,7 +177,7 @@ __global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
         double na[R], nc[R], nd[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = threadIdx.x + r * kPcrThreads;
+            const int j = threadIdx.x + r * NT;
             na[r] = 0.0; nc[r] = 0.0; nd[r] = 0.0;
             if (j < ny) {
                 const double a = A[j], c = Cc[j], d = D[j];
@@ -194,14 +193,14 @@ __global__ __launch_bounds__(kPcrThreads) void k_line_pcr(Level L, int color)
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = threadIdx.x + r * kPcrThreads;
+            const int j = threadIdx.x + r * NT;
             if (j < ny) { A[j] = na[r]; Cc[j] = nc[r]; D[j] = nd[r]; }
         }
         __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = threadIdx.x + r * kPcrThreads;
+        const int j = threadIdx.x + r * NT;
         if (j >= ny) break;
         const size_t k = (size_t)j * L.nx + i;
         if (L.act[k]) L.phi[k] = D[j];
@@ -279,12 +278,12 @@ __global__ void k_prolong(Level c, Level f)
     f.phi[k] += e;
 }
 
-template <int R>
+template <int R, int NT>
 void launch_pcr(const Level &L, int color, hipStream_t s)
 {
     const int nb = (L.nx + 1 - color) / 2;
     if (nb <= 0) return;
-    k_line_pcr<R><<<nb, kPcrThreads, 3 * L.ny * sizeof(double), s>>>(L, color);
+    k_line_pcr<R, NT><<<nb, NT, 3 * L.ny * sizeof(double), s>>>(L, color);
 }
 
 }  // namespace
@@ -334,10 +333,10 @@ void Solver::smooth(int l, int sweeps)
     const Level &L = lv_[l];
     for (int k = 0; k < sweeps; ++k)
         for (int color = 0; color < 2; ++color) {
-            if (L.ny <= 256) launch_pcr<1>(L, color, s_);
-            else if (L.ny <= 512) launch_pcr<2>(L, color, s_);
-            else if (L.ny <= 1024) launch_pcr<4>(L, color, s_);
-            else if (L.ny <= kMaxNy) launch_pcr<8>(L, color, s_);
+            if (L.ny <= 256) launch_pcr<1, 256>(L, color, s_);
+            else if (L.ny <= 512) launch_pcr<1, 512>(L, color, s_);
+            else if (L.ny <= 1024) launch_pcr<1, 1024>(L, color, s_);
+            else if (L.ny <= kMaxNy) launch_pcr<2, 1024>(L, color, s_);
             else {
                 const int ncol = (L.nx + 1 - color) / 2;
                 if (ncol > 0) k_line_thomas<<<(ncol + 63) / 64, 64, 0, s_>>>(L, color);
