@@ -306,26 +306,35 @@ __global__ void k_reduce_runs(const unsigned *keys, const double *vals, size_t c
     const unsigned key = keys[t];
     if (key == 0xffffffffu) return;
     if (t > 0 && keys[t - 1] == key) return;
-    // the same sequential sum, with the loads of 16 entries issued ahead
-    // of their additions
+    // the same sequential sum; the loads of the next G entries are in flight
+    // while the current G are added (two register buffers)
     constexpr int G = 16;
     double s = 0.0;
-    for (size_t u = t;; u += G) {
-        unsigned kv[G];
-        double vv[G];
+    unsigned ka[G], kb[G];
+    double va[G], vb[G];
+    auto fetch = [&](size_t u, unsigned (&kv)[G], double (&vv)[G]) {
 #pragma unroll
         for (int m = 0; m < G; ++m) {
             const bool in = u + m < cnt;
             kv[m] = in ? keys[u + m] : ~key;
             vv[m] = in ? vals[u + m] : 0.0;
         }
+    };
+    auto add = [&](const unsigned (&kv)[G], const double (&vv)[G]) {
         bool more = true;
 #pragma unroll
         for (int m = 0; m < G; ++m) {
             more = more && kv[m] == key;
             if (more) s += vv[m];
         }
-        if (!more) break;
+        return more;
+    };
+    fetch(t, ka, va);
+    for (size_t u = t;; u += 2 * G) {
+        fetch(u + G, kb, vb);
+        if (!add(ka, va)) break;
+        fetch(u + 2 * G, ka, va);
+        if (!add(kb, vb)) break;
     }
     const int i = (int)(key % (unsigned)f.n), j = (int)(key / (unsigned)f.n);
     double *c = comp == 0 ? f.c0 : f.c1;
