@@ -2,7 +2,7 @@
  * csrst_oracle.cpp -- CPU FP64 oracle for the CSRST particle-in-cell time step.
  *
  * TEST INFRASTRUCTURE ONLY: a plain C++ restatement of the reference algorithm
- * (kreischtauf/CSRST src/), used as the checker for the HIP product path.  It
+ * (kreischtauf/CSRST src/), used to check the HIP product path.  It
  * is never linked into, loaded by or called from the product library.
  *
  * Every function cites the reference file:line it restates.  Arithmetic is

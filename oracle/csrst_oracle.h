@@ -3,13 +3,37 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This library is a plain C++ restatement of the
  * reference algorithm (kreischtauf/CSRST, src/) and exists to check the HIP
- * product path.  Only tests/, the smoke() checker and the CPU-baseline leg of
- * bench.py may load it.  The product library never links, loads or calls it.
+ * product path.  Only tests/, __graft_entry__.smoke() and the CPU-baseline leg
+ * of bench.py may load it.  The product library never links, loads or calls it.
  *
  * Build: g++ -O3 -ffp-contract=off (no FMA contraction, like the reference's
  * ISO C++ build, CMakeLists.txt:85-87).
  *
- * Pinning status of each function is listed in oracle/README.md.
+ * What pins each function (tests/test_oracle.py, tests/test_oracle_pins.py;
+ * tests/test_oracle_mutants.py plants one mistake per pin and requires the pin
+ * to fail on it):
+ *   gather / scatter weights, IntS2 bug   the reference's golden vectors
+ *                                         (test/scattergather.cpp:90-416)
+ *   crossings                             brute-force segment sampling
+ *   S0/S1/S2 deposits                     discrete continuity identity on the
+ *                                         reference's test trajectories and
+ *                                         random ones; a uniform current
+ *   Q_rsqrt                               error bound against 1/sqrt(x)
+ *   Yee split step (bulk)                 plane waves: |lambda| = 1, fields
+ *                                         follow the mode to 1e-11
+ *   conformal cells, limitArea            walls through vertices reduce to
+ *                                         PEC; a wave beside a fractional wall
+ *   push, rotation, external field        the rotation map in the field strip;
+ *                                         uncharged orbits on their circle
+ *   exchange, moveMesh                    window = global grid under it
+ *   Poisson operator and solve            manufactured solutions; line charge
+ *                                         between plates (closed form)
+ *   beam statistics                       4-particle closed form, rotation
+ *                                         invariance
+ *   line fields (SaveEMFields)            uniform fields, line geometry
+ *   CSR wake (IntegrateElong)             parity unpinned at the IPPL IntCIC
+ *                                         gather (IPPL absent)
+ *   Distribution (GSL Halton)             parity unpinned (GSL absent)
  */
 #ifndef CSRST_ORACLE_H
 #define CSRST_ORACLE_H

@@ -1,6 +1,6 @@
 """ctypes front end of the CPU FP64 oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke() checker and by the
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
 CPU-baseline leg of bench.py.  The product package never imports this module.
 """
 import ctypes as C

@@ -2,7 +2,7 @@
 BinaryVtkFile (BinaryVTKFile.hh:25-381, BinaryVTKFile.cpp:25-217) as
 PlotBinaryVTKCommand drives it (PlotBinaryVTK.cpp:30-88, PlotBinaryVTK.hh:58-80).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the driver's
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference writer for the driver's
 writer.  The product never imports this module.
 
 The restatement follows the reference's string building step by step
