@@ -3,8 +3,9 @@
 c1, c2, c3, c5 (the production moving window, FP32 split precision and
 FP64 on the same seeded bunch): the runs finish without a window error, the
 windows move together and follow the bunch round the bend, the FP32 CSR
-wake stays within 1e-4 of the FP64 wake (c5: 1e-2, see CASES) and the FP32
-centroid and mean momentum within 1e-3 cells and 1e-6 of FP64's.  c2 and c3 start half-way round the arc.
+wake stays within 1e-4 of the FP64 wake (c5: 1e-2, see CASES), the FP32
+centroid and mean momentum within 1e-3 cells and 1e-6 of FP64's, and (c5)
+the gain of the density modulation within 1e-3 of FP64's.  c2 and c3 start half-way round the arc.
 c1 (CFL 0.5, 16 PML columns, first-order shape) and c5 start with the
 Poisson field of the bunch, which the reference computes for a bunch moving
 along the first straight section (PoissonSolver.cpp:64-71 boosts along x):
@@ -34,6 +35,22 @@ C_LIGHT = 299792458.0
 CASES = [("c1", 2000, -0.01, 1e-4), ("c2", 1000, 0.5, 1e-4), ("c3", 1000, 0.5, 1e-4), ("c5", 3000, -2e-3, 1e-2)]
 
 
+def bunching(R, P, lam, chunk=5_000_000):
+    """|<exp(i k s)>| over the particles, s the position along the mean
+    momentum from the centroid, k = 2 pi / lam (the bunching factor of the
+    density modulation at wavelength lam)."""
+    d = P.mean(0)
+    d = d / np.hypot(d[0], d[1])
+    c0 = R.mean(0)
+    k = 2 * np.pi / lam
+    re = im = 0.0
+    for a in range(0, len(R), chunk):
+        s = (R[a:a + chunk] - c0) @ d
+        re += np.cos(k * s).sum()
+        im += np.sin(k * s).sum()
+    return np.hypot(re, im) / len(R)
+
+
 def run(cfg, fp32, R, P, Q, steps, stat):
     m = MovingWindowStep(cfg["W"], cfg["H"], cfg["dx"], cfg["dy"], cfg["lenBefore"], cfg["lenAfter"], cfg["width"],
                          cfg["phiDeg"], cfg["ekinMeV"], cfg["bz"], order=cfg["order"], fp32=fp32,
@@ -44,6 +61,8 @@ def run(cfg, fp32, R, P, Q, steps, stat):
     s0 = m.calcBeamParameters()
     m.step(steps)
     out = dict(wake=m.wake(), info=m.info(), pos0=pos0, s0=s0, s1=m.calcBeamParameters(), poisson=pois)
+    if cfg.get("modulation"):
+        out["bunching"] = bunching(*m.particles(), cfg["modulation"][1])
     m.close()
     return out
 
@@ -58,6 +77,7 @@ def test_workload_in_bend(gpu, name, steps, where, wtol):
         R, P, Q = bunch_in_bend(cfg, seed=3, arc_fraction=where)
     else:
         R, P, Q = bunch_for(cfg, seed=3, s_offset=where)
+    b0 = bunching(R, P, cfg["modulation"][1]) if cfg.get("modulation") else None
     r64 = run(cfg, False, R, P, Q, steps, 20)
     r32 = run(cfg, True, R, P, Q, steps, 20)
     del R, P, Q
@@ -85,6 +105,14 @@ def test_workload_in_bend(gpu, name, steps, where, wtol):
              "; Poisson V-cycles %s" % r64["poisson"]["cycles"] if r64["poisson"] else ""), flush=True)
     assert werr < wtol
     assert cerr < 1e-3 and perr < 1e-6
+    if b0 is not None:
+        # the density modulation's gain (bunching factor after / before) in
+        # FP32 against FP64
+        g64, g32 = r64["bunching"] / b0, r32["bunching"] / b0
+        print("%s: bunching factor at %.0f um: %.5f at the start; gain FP64 %.6f, FP32 %.6f (difference %.1e)"
+              % (name, 1e6 * cfg["modulation"][1], b0, g64, g32, abs(g32 - g64) / g64), flush=True)
+        assert b0 > 0.02
+        assert abs(g32 - g64) <= 1e-3 * g64
 
 
 def test_c4_inference_and_multigrid(gpu):
