@@ -457,9 +457,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (split int32+fp32 positions)",
+        "dtype": "f32",
         "data": "synthetic seeded Gaussian bunch (Philox), analytic bend chamber",
-        "config": {"workload": workload, "cells": cells, "particles": n, "pml": cfg["pml"], "cfl": cfg["cfl"]},
+        "config": {"workload": workload, "cells": cells, "particles": n, "pml": cfg["pml"], "cfl": cfg["cfl"],
+                   "positions": "int32 cell + f32 offset"},
         "cell_updates_per_s": cu32,
         "step_roofline_frac": frac_step,
         "step_roofline_frac_48B_cells": frac_step48,
