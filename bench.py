@@ -303,6 +303,42 @@ def cpu_baseline(seconds=12.0, max_procs=16):
                                       "throughputs summed" % k))
 
 
+def copy_bandwidth(mib=1024, reps=5):
+    """Attainable HBM bandwidth: the best of `reps` device-to-device copies of
+    mib MiB (the runtime's copy kernel), timed with HIP events; read plus
+    written bytes over the time, GB/s."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    for q in (src, dst):
+        if hip.hipMalloc(C.byref(q), nbytes) != 0:
+            raise RuntimeError("copy bandwidth: hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, nbytes) != 0 or hip.hipEventCreate(C.byref(e0)) != 0 or \
+                hip.hipEventCreate(C.byref(e1)) != 0:
+            raise RuntimeError("copy bandwidth: set-up failed")
+        best = float("inf")
+        for _ in range(reps + 1):           # the first copy warms up
+            hip.hipEventRecord(e0, None)
+            hip.hipMemcpyAsync(dst, src, nbytes, 3, None)
+            hip.hipEventRecord(e1, None)
+            if hip.hipEventSynchronize(e1) != 0:
+                raise RuntimeError("copy bandwidth: copy failed")
+            ms = C.c_float(0)
+            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+            best = min(best, ms.value)
+        return 2.0 * (mib << 20) / (best * 1e-3) / 1e9
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        if e0:
+            hip.hipEventDestroy(e0)
+        if e1:
+            hip.hipEventDestroy(e1)
+
+
 def copy_probe(mib):
     """One device-to-device hipMemcpy of mib MiB (the runtime's copyBuffer
     kernel): reads and writes mib MiB, a known traffic for the HBM counters."""
@@ -442,6 +478,7 @@ def main():
         workload += ", bunch at %g deg" % a.angle
     elif a.slice == "bend":
         workload += ", bunch half-way round the bend"
+    attainable = copy_bandwidth()
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline()
@@ -470,7 +507,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "k_push (push + deposit, %d B per particle per launch)" % B_PART[True],
-                     "launch_ms": push_launch_ms, "sweep_ms_per_step": sweep_ms},
+                     "launch_ms": push_launch_ms, "sweep_ms_per_step": sweep_ms,
+                     "attainable_copy_gbs": attainable},
         "fp64": rec64,
         "cpu_baseline": cpu,
     }
