@@ -228,6 +228,25 @@ public:
         check(csrst_download_field(ctx_, which, f.data(), n, &n), ctx_);
         return f;
     }
+    // The caller's geometry in place of the one Bend computes: the inside mask
+    // (numgridpx x numgridpy bytes) and SCBoundaryCondition::getBoundaryCells()
+    // (SCBoundaryCondition.hh:39-84).  Before the PartBunch is created.
+    void setGeometry(const std::vector<uint8_t> &mask, const std::vector<BoundaryCell> &cells)
+    {
+        if (!mask.empty()) check(csrst_set_mask(ctx_, mask.data()), ctx_);
+        std::vector<int> ij(2 * cells.size());
+        std::vector<double> d(5 * cells.size());
+        for (size_t k = 0; k < cells.size(); ++k) {
+            const BoundaryCell &c = cells[k];
+            ij[2 * k] = c.i; ij[2 * k + 1] = c.j;
+            d[5 * k] = c.lambda[0]; d[5 * k + 1] = c.lambda[1]; d[5 * k + 2] = c.area;
+            d[5 * k + 3] = c.lambdaToNeighbours[0]; d[5 * k + 4] = c.lambdaToNeighbours[1];
+        }
+        check(csrst_set_boundary_cells(ctx_, ij.data(), d.data(), (int)cells.size()), ctx_);
+    }
+    // The engine's GPU work on the caller's stream (a hipStream_t; nullptr:
+    // the engine's own)
+    void setStream(void *stream) { check(csrst_set_stream(ctx_, stream), ctx_); }
 
 private:
     Commands cmd_;
@@ -313,6 +332,73 @@ private:
     Engine &e_;
     size_t n_;
     mutable double d_[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// A node patch of a mesh: nodes [i0, i0 + nx) x [j0, j0 + ny), node (0, 0) of
+// the mesh at `origin`, spacing dx.  Patch arrays are [ny][nx] doubles.
+struct NodePatch {
+    int i0 = 0, j0 = 0, nx = 0, ny = 0;
+    double origin[2] = {0.0, 0.0};
+    double dx[2] = {1.0, 1.0};
+};
+
+// ShapeFunction<IntOp> (ShapeFunction.hh:30-508) on device arrays: ORDER 0
+// IntCIC, 1 IntS1, 2 IntS2.  R, oldR: interleaved (x, y) per particle; every
+// pointer is device memory; `stream` a hipStream_t (nullptr: the null stream).
+template <int ORDER>
+struct ShapeFunction {
+    static_assert(ORDER >= 0 && ORDER <= 2, "IntCIC, IntS1 or IntS2");
+    // getCurrentDensity{X,Y} with its Base part: J over the patch (overwritten)
+    static void getCurrentDensity(int comp, double *J, const NodePatch &p, double dt, const double *oldR,
+                                  const double *R, const double *Q, int n, void *stream = nullptr)
+    {
+        check(csrst_sf_current_density(ORDER, comp, J, p.i0, p.j0, p.nx, p.ny, p.origin, p.dx, dt, oldR, R, Q, n,
+                                       stream));
+    }
+    static void getCurrentDensityX(double *J, const NodePatch &p, double dt, const double *oldR, const double *R,
+                                   const double *Q, int n, void *stream = nullptr)
+    {
+        getCurrentDensity(0, J, p, dt, oldR, R, Q, n, stream);
+    }
+    static void getCurrentDensityY(double *J, const NodePatch &p, double dt, const double *oldR, const double *R,
+                                   const double *Q, int n, void *stream = nullptr)
+    {
+        getCurrentDensity(1, J, p, dt, oldR, R, Q, n, stream);
+    }
+    // getChargeDensity: rho over the patch (overwritten)
+    static void getChargeDensity(double *rho, const NodePatch &p, const double *R, const double *Q, int n,
+                                 void *stream = nullptr)
+    {
+        check(csrst_sf_charge_density(ORDER, rho, p.i0, p.j0, p.nx, p.ny, p.origin, p.dx, R, Q, n, stream));
+    }
+    // getChargeDensityDiff: rho(R) - rho(oldR)
+    static void getChargeDensityDiff(double *rho, const NodePatch &p, const double *R, const double *oldR,
+                                     const double *Q, int n, void *stream = nullptr)
+    {
+        check(csrst_sf_charge_density_diff(ORDER, rho, p.i0, p.j0, p.nx, p.ny, p.origin, p.dx, R, oldR, Q, n,
+                                           stream));
+    }
+};
+
+// ScatterGatherImpl<IntOp, double> (ScatterGatherImpl.hh:27-64) on device
+// arrays: a field on the nodes of an nx x ny patch with node (0, 0) at
+// `origin`.  Edge fields take comp 0 (x edges) or 1 (y edges); cell fields
+// comp -1.  s2bug: the reference's (I+1, J+2) term of the IntS2 gather.
+template <int ORDER>
+struct ScatterGatherImpl {
+    static_assert(ORDER >= 0 && ORDER <= 2, "IntCIC, IntS1 or IntS2");
+    static void gather(const double *field, int nx, int ny, const double origin[2], const double dx[2], int comp,
+                       const double *R, int n, double *out, bool s2bug = true, void *stream = nullptr)
+    {
+        check(csrst_sg_gather(ORDER, s2bug ? 1 : 0, comp < 0 ? 2 : comp, field, nx, ny, origin, dx, R, n, out,
+                              stream));
+    }
+    // field += the particles' values times their weights
+    static void scatter(double *field, int nx, int ny, const double origin[2], const double dx[2], int comp,
+                        const double *R, const double *val, int n, void *stream = nullptr)
+    {
+        check(csrst_sg_scatter(ORDER, comp < 0 ? 2 : comp, field, nx, ny, origin, dx, R, val, n, stream));
+    }
 };
 
 // PartBunchState (PartBunch.hh:290-323): what PartBunch::calcBeamParameters
