@@ -67,6 +67,7 @@ struct WinState {
     int err;             // 1: stencil left the window, 2: too many crossings, 4: move out of range
     int steps;
     double p0[2];        // reference momentum: particles store P - p0 (re-centred at every sort)
+    int sbox[4];         // cell box of the particles at the last sort: min x, min y, -max x, -max y
 };
 
 template <class T>
@@ -92,6 +93,31 @@ struct Parts {
     T *fx, *fy, *px, *py;
     int n;
 };
+
+// Buffer resource over one field plane (bytes, below 4 GiB: the host refuses
+// larger windows), for loads at 32-bit byte offsets.  A run of loads at
+// constant distances from one offset takes immediate offsets and merges into
+// wide loads; an offset past the plane reads 0.  Word 3 is the gfx9 family's
+// raw 32-bit data format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *plane, size_t elems, size_t esize)
+{
+    const size_t bytes = elems * esize;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(plane), (short)0,
+                                             (int)(uint32_t)(bytes < 0xffffffffu ? bytes : 0xffffffffu), 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const T *plane, size_t elems)
+{
+    return plane_rsrc(plane, elems, sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
 
 struct Geo {
     geom::Bend bend;
@@ -468,6 +494,8 @@ constexpr int kRun = 4;              // consecutive particles per vector load (1
 constexpr int kPPT = 4;              // particles per thread: one run of kRun consecutive sorted particles
 constexpr int kChunk = kB * kPPT;    // 1024 particles per workgroup
 constexpr int kTW = 64, kTH = 32;    // LDS accumulation tile (nodes)
+constexpr int kSlots = 2;            // register accumulation slots per lane (one stencil anchor each)
+constexpr int kMaxSub = 8;           // sort key: at most 2^8 x 2^8 bins per cell
 
 // Particle arrays start at multiples of 64 elements (Parts stride), so a run
 // of kRun values at a multiple of kRun is one aligned vector access.
@@ -513,7 +541,7 @@ __device__ __forceinline__ void rotate_in(T (&o)[kRun], T v)
 // Push of one workgroup chunk (kChunk sorted particles): thread t owns
 // particles base + kPPT t .. + kPPT - 1, so its consecutive particles share a
 // cell and their segment currents share stencil anchors.  Currents collect in
-// two register slots per lane (one anchor each).  A segment whose anchor
+// kSlots register slots per lane (one anchor each).  A segment whose anchor
 // matches neither (a run crossing a cell boundary, or a bunch moving at an
 // angle, whose runs have up to four anchors) goes into the chunk's LDS tile;
 // at the end the slots are reduced across the wavefront once per distinct
@@ -639,12 +667,16 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
         coop_add(mine, sc.I0, sc.J0, sc.w);
         qn = 0;
     };
-    // two register accumulation slots per lane
-    bool hasA = false, hasB = false;
-    int aI = 0, aJ = 0, bI = 0, bJ = 0;
-    T accA[NN], accB[NN];
+    // kSlots register accumulation slots per lane
+    bool has[kSlots];
+    int sI[kSlots], sJ[kSlots];
+    T sacc[kSlots][NN];
 #pragma unroll
-    for (int t = 0; t < NN; ++t) { accA[t] = T(0); accB[t] = T(0); }
+    for (int q = 0; q < kSlots; ++q) {
+        has[q] = false; sI[q] = 0; sJ[q] = 0;
+#pragma unroll
+        for (int t = 0; t < NN; ++t) sacc[q][t] = T(0);
+    }
 #pragma unroll 1
     for (int r = 0; r < kPPT / kRun; ++r) {
         const int k0 = kt + r * kRun;
@@ -685,29 +717,66 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
                 // offsets from the uniform plane base).
                 const vec2<T> pp = pm::mk(fx, fy);
                 constexpr int N = ORDER + 2, L = ORDER == 2 ? -2 : -1, U = N + 1;
-                uint32_t C[U], R[U];
+                // The union's first column and row, clamped so that all U
+                // stay inside the guarded window (a particle outside it has
+                // set the error bit; its loads stay in bounds), then the ring
+                // offsets: rows one by one (each may wrap), columns as one
+                // base when the U columns do not cross the ring's seam, so a
+                // row's loads take immediate offsets from one address.
+                uint32_t R[U], C0;
+                bool cwrap;
                 {
                     const int i0 = gx + L - wx0, j0 = gy + L - wy0;
                     if (i0 < -G || i0 + U - 1 > W - 1 + G || j0 < -G || j0 + U - 1 > H - 1 + G) err |= 1;
+                    int px = min(max(i0, -G), W - U + G) + G + rx;
+                    px = px >= f.Wp ? px - f.Wp : px;
+                    cwrap = px > f.Wp - U;
+                    C0 = (uint32_t)px * (uint32_t)sizeof(T);
+                    int py = min(max(j0, -G), H - U + G) + G + ry;
+                    py = py >= f.Hp ? py - f.Hp : py;
+                    const uint32_t rowb = (uint32_t)(f.Wp * sizeof(T));
+                    R[0] = (uint32_t)py * rowb;
 #pragma unroll
-                    for (int a = 0; a < U; ++a) {
-                        int px = min(max(i0 + a, -G), W - 1 + G) + G + rx;
-                        px = px >= f.Wp ? px - f.Wp : px;
-                        C[a] = (uint32_t)px * (uint32_t)sizeof(T);
-                        int py = min(max(j0 + a, -G), H - 1 + G) + G + ry;
-                        py = py >= f.Hp ? py - f.Hp : py;
-                        R[a] = (uint32_t)py * (uint32_t)(f.Wp * sizeof(T));
+                    for (int a = 1; a < U; ++a) {
+                        const bool wrap = ++py == f.Hp;
+                        py = wrap ? 0 : py;
+                        R[a] = wrap ? 0u : R[a - 1] + rowb;
                     }
                 }
+                const bool anyWrap = __any(cwrap);
                 auto gat = [&](const T *plane, const pm::Stencil<T> &st) -> T {
                     const bool sx = st.I0 != L, sy = st.J0 != L;
-                    uint32_t cc[N], rr[N];
+                    uint32_t rr[N];
 #pragma unroll
-                    for (int a = 0; a < N; ++a) { cc[a] = sx ? C[a + 1] : C[a]; rr[a] = sy ? R[a + 1] : R[a]; }
-                    const char *pb = reinterpret_cast<const char *>(plane);
-                    return pm::gather_n<N>(st, false, [&](int a, int b) {
-                        return *reinterpret_cast<const T *>(pb + (rr[b] + cc[a]));
-                    });
+                    for (int a = 0; a < N; ++a) rr[a] = sy ? R[a + 1] : R[a];
+                    // The empty asm makes the plane pointer opaque here, so the
+                    // resource (four scalar registers) is built at each gather
+                    // instead of three being hoisted out of the particle loop,
+                    // where they push other scalars into spills (each reload
+                    // is a VALU readlane).
+                    const T *pl = plane;
+                    asm volatile("" : "+s"(pl));
+                    const auto rs = plane_rsrc(pl, f.plane);
+                    if (!anyWrap) {
+                        // one offset per row; its N columns are consecutive
+                        // (immediate offsets, merged into wide loads)
+                        const uint32_t cb = C0 + (sx ? (uint32_t)sizeof(T) : 0u);
+                        return pm::gather_n<N>(st, false, [&](int a, int b) {
+                            return buf_load<T>(rs, rr[b] + cb + (uint32_t)(a * sizeof(T)));
+                        });
+                    }
+                    // a stencil of this wavefront crosses the seam: per-column offsets
+                    uint32_t cc[N];
+                    {
+                        int px = (int)(C0 / (uint32_t)sizeof(T)) + (sx ? 1 : 0);
+#pragma unroll
+                        for (int a = 0; a < N; ++a) {
+                            px = px >= f.Wp ? px - f.Wp : px;
+                            cc[a] = (uint32_t)px * (uint32_t)sizeof(T);
+                            ++px;
+                        }
+                    }
+                    return pm::gather_n<N>(st, false, [&](int a, int b) { return buf_load<T>(rs, rr[b] + cc[a]); });
                 };
                 T ex, ey, hf;
                 if constexpr (ORDER == 2) {
@@ -789,31 +858,31 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
                     const T tau = s == 0 ? xs.f[1] - xs.f[0] : xs.f[2] - xs.f[1];
                     seg_current(sc, xi, xf, tau, delta, gx0, gy0);
                 }
-                // register accumulation by anchor; a segment matching neither
-                // slot goes to the LDS tile (MISSCOOP: through the wavefront)
-                bool miss = false;
-                if (act) {
-                    if (hasA && sc.I0 == aI && sc.J0 == aJ) {
+                // register accumulation by anchor: the first kSlots anchors a
+                // lane meets own its slots, in order (their sums start at
+                // zero); a segment matching none goes to the LDS tile
+                // (MISSCOOP: through the wavefront).  Independent conditional
+                // adds: a wavefront whose lanes take different slots pays one
+                // pass per slot taken.
+                bool taken = !act;
 #pragma unroll
-                        for (int t = 0; t < NN; ++t) accA[t] += sc.w[t];
-                    } else if (hasB && sc.I0 == bI && sc.J0 == bJ) {
+                for (int q = 0; q < kSlots; ++q) {
+                    const bool to = !taken && (!has[q] || (sc.I0 == sI[q] && sc.J0 == sJ[q]));
+                    sI[q] = to ? sc.I0 : sI[q];
+                    sJ[q] = to ? sc.J0 : sJ[q];
+                    has[q] = has[q] || to;
+                    if (to) {
 #pragma unroll
-                        for (int t = 0; t < NN; ++t) accB[t] += sc.w[t];
-                    } else if (!hasA) {
-                        hasA = true; aI = sc.I0; aJ = sc.J0;
-#pragma unroll
-                        for (int t = 0; t < NN; ++t) accA[t] = sc.w[t];
-                    } else if (!hasB) {
-                        hasB = true; bI = sc.I0; bJ = sc.J0;
-#pragma unroll
-                        for (int t = 0; t < NN; ++t) accB[t] = sc.w[t];
-                    } else if (MISSCOOP) {
-                        miss = true;
-                    } else {
-                        lane_add(sc.I0, sc.J0, sc.w);
+                        for (int t = 0; t < NN; ++t) sacc[q][t] += sc.w[t];
                     }
+                    taken = taken || to;
                 }
-                if (MISSCOOP && __any(miss)) coop_add(miss, sc.I0, sc.J0, sc.w);
+                const bool miss = !taken;
+                if (MISSCOOP) {
+                    if (__any(miss)) coop_add(miss, sc.I0, sc.J0, sc.w);
+                } else if (miss) {
+                    lane_add(sc.I0, sc.J0, sc.w);
+                }
             }
             // ---- drift back by half a step (TETMSolver.hh:210, 265)
             if (live) {
@@ -838,11 +907,21 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(1, 3)))
     // ---- queued third segments, then the register slots
     if (qn) drain_queue();
 #pragma unroll 1
-    for (int slot = 0; slot < 2; ++slot) {
+    for (int slot = 0; slot < kSlots; ++slot) {
         T w[NN];
+        bool h = has[0];
+        int I = sI[0], Jn = sJ[0];
 #pragma unroll
-        for (int t = 0; t < NN; ++t) w[t] = slot == 0 ? accA[t] : accB[t];
-        coop_add(slot == 0 ? hasA : hasB, slot == 0 ? aI : bI, slot == 0 ? aJ : bJ, w);
+        for (int t = 0; t < NN; ++t) w[t] = sacc[0][t];
+#pragma unroll
+        for (int q = 1; q < kSlots; ++q) {
+            h = slot == q ? has[q] : h;
+            I = slot == q ? sI[q] : I;
+            Jn = slot == q ? sJ[q] : Jn;
+#pragma unroll
+            for (int t = 0; t < NN; ++t) w[t] = slot == q ? sacc[q][t] : w[t];
+        }
+        coop_add(h, I, Jn, w);
     }
     // ---- chunk's particle box and error bits, then flush the tile with one
     // global atomic per nonzero node
@@ -1274,18 +1353,80 @@ __global__ void k_pois_shift_guess(const double *phi, double *out, int nx, int n
 // ---------------------------------------------------------------------------
 // cell sort keys and permutation
 template <class T>
-__global__ void k_sort_keys(Parts<T> p, const WinState *ws, int Wk, unsigned *keys, int *idx)
+__global__ void k_sort_keys(Parts<T> p, const WinState *ws, unsigned *keys, int *idx)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= p.n) return;
-    const int i = p.gx[k] - ws->wx0 + 8, j = p.gy[k] - ws->wy0 + 8;
-    const unsigned tile = (unsigned)(j >> 2) * (unsigned)(Wk >> 4) + (unsigned)(i >> 4);
-    // lowest bits: the half of the cell the particle is in.  The second push
-    // of a step starts half a step on, where a cell's particles have split by
-    // these halves; runs of consecutive particles then still share a cell.
-    const unsigned hx = p.fx[k] >= T(0.5) ? 1u : 0u, hy = p.fy[k] >= T(0.5) ? 1u : 0u;
-    keys[k] = (tile << 8) | ((unsigned)(j & 3) << 6) | ((unsigned)(i & 15) << 2) | (hy << 1) | hx;
+    // 16x4-cell tiles over the particles' cell box, row-major; the bits the
+    // tile index leaves of 32 (after 6 of cell in the tile) go to the
+    // particle's bin in its cell, 2^sb x 2^sb bins (sb <= kMaxSub).
+    // Particles move nearly together, so consecutive particles of one bin
+    // stay in one cell, or two, for many pushes after the sort: the finer
+    // the bins, the fewer runs of a push straddle a cell line.
+    const int x0 = ws->sbox[0], y0 = ws->sbox[1];
+    const unsigned tw = (unsigned)(-ws->sbox[2] - x0 + 16) >> 4;      // tiles per row
+    const unsigned tiles = tw * ((unsigned)(-ws->sbox[3] - y0 + 4) >> 2);
+    const int tb = 32 - __clz((int)(tiles > 1u ? tiles - 1u : 1u));  // bits of the largest tile index
+    const int sb = max(1, min(kMaxSub, (32 - 6 - tb) / 2));
+    const int i = p.gx[k] - x0, j = p.gy[k] - y0;
+    const unsigned tile = (unsigned)(j >> 2) * tw + (unsigned)(i >> 4);
+    const T sc = T(1 << sb);
+    const unsigned bx = (unsigned)(p.fx[k] * sc), by = (unsigned)(p.fy[k] * sc);   // fx, fy in [0, 1)
+    // the quadrant of the cell (the bins' top bits) ranks first: the second
+    // push of a step starts half a step on, where a cell's particles have
+    // split by halves, so consecutive runs of one quadrant share a cell then
+    const int r = sb - 1;
+    const unsigned lo = (1u << r) - 1u;
+    const unsigned bins = ((by >> r) << (2 * sb - 1)) | ((bx >> r) << (2 * sb - 2)) | ((by & lo) << r) | (bx & lo);
+    keys[k] = (tile << (6 + 2 * sb)) | ((unsigned)(j & 3) << (4 + 2 * sb)) | ((unsigned)(i & 15) << (2 * sb)) | bins;
     idx[k] = k;
+}
+
+// Cell box of the particles (min x, min y, -max x, -max y) for the sort
+// keys: per-block minima, then one block reduces them into ws->sbox.
+template <class T>
+__global__ void __launch_bounds__(kB) k_box_partial(Parts<T> p, int *partial)
+{
+    __shared__ int sh[kB / 64][4];
+    int b[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+    for (int k = blockIdx.x * kB + threadIdx.x; k < p.n; k += gridDim.x * kB) {
+        const int x = p.gx[k], y = p.gy[k];
+        b[0] = min(b[0], x); b[1] = min(b[1], y); b[2] = min(b[2], -x); b[3] = min(b[3], -y);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = hu::wave_min(b[c]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh[threadIdx.x >> 6][c] = b[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int v = INT_MAX;
+        for (int w = 0; w < kB / 64; ++w) v = min(v, sh[w][threadIdx.x]);
+        partial[blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_box_final(const int *partial, int nb, WinState *ws)
+{
+    __shared__ int sh[kB / 64][4];
+    int b[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+    for (int q = threadIdx.x; q < nb; q += kB) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = min(b[c], partial[q * 4 + c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[c] = hu::wave_min(b[c]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh[threadIdx.x >> 6][c] = b[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int v = INT_MAX;
+        for (int w = 0; w < kB / 64; ++w) v = min(v, sh[w][threadIdx.x]);
+        ws->sbox[threadIdx.x] = v;
+    }
 }
 
 template <class T>
@@ -1640,8 +1781,10 @@ template <class T>
 void sort_particles(csrst_mw *m)
 {
     const int nb = (m->n + mw::kB - 1) / mw::kB;
-    const int Wk = (m->W + 16 + 15) & ~15;
-    mw::k_sort_keys<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->ws.p, Wk, m->keys.p, m->idx.p);
+    int *boxPart = reinterpret_cast<int *>(m->partial.p);   // kRedBlocks x 4 ints
+    mw::k_box_partial<T><<<mw::kRedBlocks, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), boxPart);
+    mw::k_box_final<<<1, mw::kB, 0, m->s>>>(boxPart, mw::kRedBlocks, m->ws.p);
+    mw::k_sort_keys<T><<<nb, mw::kB, 0, m->s>>>(m->parts<T>(m->partBuf), m->ws.p, m->keys.p, m->idx.p);
     HU_LAUNCH_CHECK();
     HU_CHECK(hipcub::DeviceRadixSort::SortPairs(m->cubTmp.p, m->cubBytes, m->keys.p, m->keys2.p, m->idx.p, m->idx2.p,
                                                 m->n, 0, 32, m->s));
@@ -1850,8 +1993,10 @@ int csrst_mw_create(const csrst_mw_params *p, csrst_mw **out)
         m->fieldsBuf.alloc(m->plane * 12 * rs);   // ex ey hz jx jy ex' hz' + 5 cut planes
         m->fieldsBuf.zero(m->s);
         if (p->pml < 0 || 2 * p->pml > p->W - 8) throw Error(CSRST_EINVAL, "pml must be in [0, (W - 8) / 2]");
-        // sort keys: 24 bits of 16x4-cell tile index above 8 bits of cell and half-cell
-        if ((long long)((p->H + 16 + 3) / 4) * (long long)((p->W + 16 + 15) / 16) >= (1ll << 24))
+        // sort keys: a 16x4-cell tile index over the particles' box (at most
+        // the window plus its guards) above 6 bits of cell in the tile and at
+        // least one bit per axis of position in the cell
+        if ((long long)((p->H + 16 + 3) / 4) * (long long)((p->W + 16 + 15) / 16) > (1ll << 24))
             throw Error(CSRST_EINVAL, "window too large for the 32-bit particle sort key");
         m->npml = p->pml;
         if (m->npml) {
