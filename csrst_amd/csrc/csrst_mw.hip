@@ -445,7 +445,12 @@ __device__ __forceinline__ void seg_s1_sep(pm::SegCurrent<T> &o, int comp, T tau
 // of the quadratic B-spline weights along the current and the cubic ones
 // across it, positions moving linearly over the segment.  The integrand has
 // degree 5 in s, so 3-point Gauss-Legendre quadrature gives the same integral
-// exactly; it costs three weight evaluations and three outer products.
+// exactly: the centre tau/2 (weight 4/9) and the pair tau (1/2 -+ r), r =
+// sqrt(15)/10 (5/18 each).  The weights are polynomials, so at the pair they
+// are their values at the centre's offsets am -+ da, bm -+ db: along Ab -+ D,
+// across S -+ U (Taylor expansions, exact), and the pair contributes
+// 2 (5/18) (Ab (x) S + D (x) U).  One evaluation at the centre and its
+// offsets replace three evaluations; the three outer products remain.
 template <class T>
 __device__ __forceinline__ void seg_s2_gauss(pm::SegCurrent<T> &o, int comp, T tau, vec2<T> a, vec2<T> b,
                                              vec2<T> qodt, vec2<T> d)
@@ -453,30 +458,38 @@ __device__ __forceinline__ void seg_s2_gauss(pm::SegCurrent<T> &o, int comp, T t
     const T al = comp == 0 ? a.x : a.y, dl = comp == 0 ? d.x : d.y;
     const T bt = comp == 0 ? b.y : b.x, dt = comp == 0 ? d.y : d.x;
     const T k = (comp == 0 ? qodt.y : qodt.x) * dl * tau;
-    // nodes and weights on [0, 1]: 1/2 -+ sqrt(15)/10, 1/2; 5/18, 4/9, 5/18
-    const T xg[3] = {T(0.5) - T(0.38729833462074168852), T(0.5), T(0.5) + T(0.38729833462074168852)};
-    const T wg[3] = {T(5.0 / 18.0), T(4.0 / 9.0), T(5.0 / 18.0)};
     const T o6 = T(1.0 / 6.0);
+    const T ht = T(0.5) * tau, rt = T(0.38729833462074168852) * tau;
+    const T am = al - dl * ht, bm = bt - dt * ht;
+    const T da = dl * rt, db = dt * rt;
+    // along: W2 = (1/2 (1/2 + a)^2, 3/4 - a^2, 1/2 (1/2 - a)^2) at am (A) and
+    // the pair's mean (Ab) and half difference (D), scaled by the quadrature
+    // weights and k
+    const T P = T(0.5) + am, M = T(0.5) - am;
+    const T P2 = P * P, M2 = M * M, c0 = T(0.75) - am * am, d2 = da * da;
+    const T kc = k * T(4.0 / 9.0), ks = k * T(5.0 / 9.0);
+    const T hc = T(0.5) * kc, hs = T(0.5) * ks, dks = ks * da;
+    const T A[3] = {hc * P2, kc * c0, hc * M2};
+    const T Ab[3] = {hs * (P2 + d2), ks * (c0 - d2), hs * (M2 + d2)};
+    const T D[3] = {dks * P, T(-2) * (dks * am), -(dks * M)};
+    // across: W3 = ((1+b)^3/6, (4 - 6b^2 - 3b^3)/6, 1 - the others, -b^3/6) at
+    // bm (B), the pair's mean (S: B + W3'' db^2 / 2) and half difference
+    // (U: W3' db + W3''' db^3 / 6)
+    const T c1 = T(1) + bm, c2 = c1 * c1, b2 = bm * bm;
+    const T B0 = o6 * (c2 * c1), B3 = -o6 * (b2 * bm), B1 = o6 * (T(4) - T(3) * b2 * (T(2) + bm));
+    const T B[4] = {B0, B1, T(1) - (B0 + B1 + B3), B3};
+    const T h = T(0.5) * (db * db);
+    const T S0 = B0 + h * c1, S3 = B3 - h * bm, S1 = B1 - h * (T(2) + T(3) * bm);
+    const T S[4] = {S0, S1, T(1) - (S0 + S1 + S3), S3};
+    const T d3 = o6 * (db * db * db);
+    const T U0 = T(0.5) * c2 * db + d3, U3 = -(T(0.5) * b2 * db) - d3;
+    const T U1 = -((T(2) * bm + T(1.5) * b2) * db) - T(3) * d3;
+    const T U[4] = {U0, U1, -(U0 + U1 + U3), U3};
     T w[12];
 #pragma unroll
-    for (int g = 0; g < 3; ++g) {
-        const T sg = tau * xg[g];
-        const T ag = al - dl * sg, bg = bt - dt * sg;
-        const T kh = k * wg[g] * T(0.5);
-        const T p = T(0.5) + ag, m = T(0.5) - ag;
-        const T A[3] = {kh * (p * p), kh * (T(1.5) - T(2) * (ag * ag)), kh * (m * m)};
-        const T b1 = T(1) + bg, bb2 = bg * bg;
-        const T B0 = o6 * (b1 * b1 * b1), B3 = -o6 * (bb2 * bg);
-        const T B1 = o6 * (T(4) - T(3) * bb2 * (T(2) + bg));
-        const T B[4] = {B0, B1, T(1) - (B0 + B1 + B3), B3};
+    for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < 3; ++ii) {
-                const T v = A[ii] * B[jj];
-                w[jj * 3 + ii] = g == 0 ? v : w[jj * 3 + ii] + v;
-            }
-    }
+        for (int ii = 0; ii < 3; ++ii) w[jj * 3 + ii] = A[ii] * B[jj] + Ab[ii] * S[jj] + D[ii] * U[jj];
     if (comp == 0) {
         o.nx = 3; o.ny = 4;
 #pragma unroll
