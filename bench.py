@@ -451,9 +451,16 @@ def main():
         print("bench: fp64 %.3f s, timing %s" % (t64, tim64), file=sys.stderr, flush=True)
         scale = np.abs(wake64).max()
         werr = float(np.abs(wake32 - wake64).max() / scale) if scale > 0 else None
+        push64_ms = tim64["push_ms"] / 2.0
         rec64 = dict(particle_steps_per_s=n * a.steps / t64, cell_updates_per_s=cells * a.steps / t64,
                      ms_per_step=1e3 * t64 / a.steps, speedup_fp32_over_fp64=t64 / t32, wake_rel_err=werr,
-                     window_moves_match=info64["pos"] == info32["pos"], timing_ms=tim64)
+                     window_moves_match=info64["pos"] == info32["pos"], timing_ms=tim64,
+                     push_launch_ms=push64_ms,
+                     push_roofline_frac=n * B_PART[False] / (push64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     step_roofline_frac=(cells * B_CELL[False] + n * B_PART[False]) * a.steps / t64 / 1e9
+                     / HBM_PEAK_GBS,
+                     step_roofline_frac_96B_cells=(cells * 96 + n * B_PART[False]) * a.steps / t64 / 1e9
+                     / HBM_PEAK_GBS)
     other = None
     if a.slice == "bend" and not a.angle and not a.no_straight:
         del R, P, Q
