@@ -1365,6 +1365,16 @@ __global__ void k_pois_shift_guess(const double *phi, double *out, int nx, int n
 
 // ---------------------------------------------------------------------------
 // cell sort keys and permutation
+// The low 8 bits of v moved to the even bit positions 0, 2, .., 14.
+__device__ __forceinline__ unsigned spread_bits(unsigned v)
+{
+    v &= 0xffu;
+    v = (v | (v << 4)) & 0x0f0fu;
+    v = (v | (v << 2)) & 0x3333u;
+    v = (v | (v << 1)) & 0x5555u;
+    return v;
+}
+
 template <class T>
 __global__ void k_sort_keys(Parts<T> p, const WinState *ws, unsigned *keys, int *idx)
 {
@@ -1385,12 +1395,12 @@ __global__ void k_sort_keys(Parts<T> p, const WinState *ws, unsigned *keys, int 
     const unsigned tile = (unsigned)(j >> 2) * tw + (unsigned)(i >> 4);
     const T sc = T(1 << sb);
     const unsigned bx = (unsigned)(p.fx[k] * sc), by = (unsigned)(p.fy[k] * sc);   // fx, fy in [0, 1)
-    // the quadrant of the cell (the bins' top bits) ranks first: the second
-    // push of a step starts half a step on, where a cell's particles have
-    // split by halves, so consecutive runs of one quadrant share a cell then
-    const int r = sb - 1;
-    const unsigned lo = (1u << r) - 1u;
-    const unsigned bins = ((by >> r) << (2 * sb - 1)) | ((bx >> r) << (2 * sb - 2)) | ((by & lo) << r) | (bx & lo);
+    // Bins in Z order (y and x bits interleaved, y above x): the quadrant of
+    // the cell ranks first (the second push of a step starts half a step on,
+    // where a cell's particles have split by halves), and at any density a
+    // run of consecutive particles fills a small square of its quadrant
+    // rather than a thin row across it.
+    const unsigned bins = (spread_bits(by) << 1) | spread_bits(bx);
     keys[k] = (tile << (6 + 2 * sb)) | ((unsigned)(j & 3) << (4 + 2 * sb)) | ((unsigned)(i & 15) << (2 * sb)) | bins;
     idx[k] = k;
 }
