@@ -983,7 +983,12 @@ struct SweepConst {
     T ihx, ihy;          // 1/hx, 1/hy
 };
 
-constexpr int kStrip = 32;
+// Rows per workgroup of the sweeps.  Short strips give many workgroups to
+// hide HBM latency, which outweighs k_heh's re-read of the rows above and
+// below its strip: at c3 the two sweeps took 1.01 ms per step with 32-row
+// strips, 0.91 ms with 8 and 0.89 ms with 4 (c2: 0.198, 0.160 ms with 4).
+// k_ey loads its rows in one group of four.
+constexpr int kStrip = 4;
 
 template <class T>
 __device__ __forceinline__ T h_update(const Fields<T> &f, size_t q, uint8_t cls, T h, T ey, T eyp, T ex, T exn,
