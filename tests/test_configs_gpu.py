@@ -117,7 +117,7 @@ def test_workload_in_bend(gpu, name, steps, where, wtol):
 
 def test_c4_inference_and_multigrid(gpu):
     """The network's guess for 2048 x 1024 densities at full size, then the
-    multigrid from that guess to the 1e-10 residual (the fallback and judge:
+    multigrid from that guess to the 1e-10 residual (the fallback and arbiter:
     it converges from any guess)."""
     from csrst_amd import poisson as PS
     from csrst_amd.warmstart import WarmStart
