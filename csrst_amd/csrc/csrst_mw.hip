@@ -1684,7 +1684,7 @@ struct csrst_mw {
     int nsWake = 0;
     int steps = 0;
     int lead = 32;
-    hipEvent_t ev[12];
+    hipEvent_t ev[16];          // two slots of a step's eight timing events (csrst_mw_step alternates)
     double lastMs[5] = {0, 0, 0, 0, 0};
     bool ready = false;
 
@@ -1837,8 +1837,9 @@ void wake(csrst_mw *m)
 }
 
 template <class T>
-void one_step(csrst_mw *m, bool timed)
+void one_step(csrst_mw *m, bool timed, int slot)
 {
+    hipEvent_t *ev = m->ev + 8 * slot;
     mw::SweepConst<T> sc;
     sweep_consts(m, sc);
     const mw::Fields<T> f = m->fields<T>();
@@ -1850,33 +1851,33 @@ void one_step(csrst_mw *m, bool timed)
     constexpr int kPmlStrip = 8;
     const dim3 gp((2 * np + 63) / 64, (m->H + kPmlStrip - 1) / kPmlStrip);
     if (m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0) {
-        if (timed) HU_CHECK(hipEventRecord(m->ev[6], m->s));
+        if (timed) HU_CHECK(hipEventRecord(ev[6], m->s));
         sort_particles<T>(m);
-        if (timed) HU_CHECK(hipEventRecord(m->ev[7], m->s));
+        if (timed) HU_CHECK(hipEventRecord(ev[7], m->s));
     }
-    if (timed) HU_CHECK(hipEventRecord(m->ev[0], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[0], m->s));
     if (m->n > 0) push_dispatch<T, 0>(m);
-    if (timed) HU_CHECK(hipEventRecord(m->ev[1], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[1], m->s));
     hipLaunchKernelGGL((mw::k_heh<T, false, mw::kStrip>), gs, dim3(mw::kB), 0, m->s, f, sc, m->ws.p, 0, np, nin);
     if (np) hipLaunchKernelGGL((mw::k_heh<T, true, kPmlStrip>), gp, dim3(64), 0, m->s, f, sc, m->ws.p, 0, 0, 2 * np);
     HU_LAUNCH_CHECK();
     m->cur ^= 1;
     const mw::Fields<T> f2 = m->fields<T>();
-    if (timed) HU_CHECK(hipEventRecord(m->ev[2], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[2], m->s));
     if (m->n > 0) push_dispatch<T, 1>(m);
-    if (timed) HU_CHECK(hipEventRecord(m->ev[3], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[3], m->s));
     hipLaunchKernelGGL((mw::k_ey<T, false, mw::kStrip>), gs, dim3(mw::kB), 0, m->s, f2, sc, m->ws.p, np, nin);
     if (np) hipLaunchKernelGGL((mw::k_ey<T, true, kPmlStrip>), gp, dim3(64), 0, m->s, f2, sc, m->ws.p, 0, 2 * np);
     HU_LAUNCH_CHECK();
     if (m->n > 0 && m->p.statModulus > 0 && m->steps % m->p.statModulus == m->p.statModulus - 1) wake<T>(m);
-    if (timed) HU_CHECK(hipEventRecord(m->ev[4], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[4], m->s));
     if (m->n > 0) {
         mw::k_move<<<1, 64, 0, m->s>>>(m->ws.p, m->W, m->H, m->Wp, m->Hp, m->lead);
         const int nf = 2 * (m->H + 2 * mw::G) + m->W + 2 * mw::G;
         hipLaunchKernelGGL((mw::k_fresh<T>), dim3((nf + mw::kB - 1) / mw::kB), dim3(mw::kB), 0, m->s, f2, m->geo, m->ws.p);
         HU_LAUNCH_CHECK();
     }
-    if (timed) HU_CHECK(hipEventRecord(m->ev[5], m->s));
+    if (timed) HU_CHECK(hipEventRecord(ev[5], m->s));
     ++m->steps;
 }
 
@@ -2188,36 +2189,52 @@ int csrst_mw_step(csrst_mw *m, int nsteps)
     try {
         if (!m || !m->ready) throw Error(CSRST_ESTATE, "csrst_mw_step before csrst_mw_init_bunch");
         double acc[5] = {0, 0, 0, 0, 0};
-        int nsorted = 0;
-        for (int k = 0; k < nsteps; ++k) {
-            const bool sorted = m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0;
-            if (m->n > 0 && m->tunePhase == 0 && m->steps >= m->nextTune) {
-                m->tunePhase = 1;
-                m->tilted = true;
-            }
-            if (m->f32) one_step<float>(m, true); else one_step<double>(m, true);
-            HU_CHECK(hipEventSynchronize(m->ev[5]));
+        // Step k records its timing events in slot k % 2, and the host reads
+        // them after it has queued step k + 1, so the GPU never idles while
+        // the host waits.  The comparison of the two slot-miss forms (a
+        // cooperative trial step, then a per-lane one) therefore decides one
+        // step after the per-lane trial ends; that step keeps the per-lane form.
+        int role[2] = {0, 0};              // per slot: 0 none, 1 cooperative trial, 2 per-lane trial
+        bool sorted[2] = {false, false};
+        auto read = [&](int b) {
+            hipEvent_t *e = m->ev + 8 * b;
+            HU_CHECK(hipEventSynchronize(e[5]));
             float t01, t12, t23, t34, t45, t67 = 0;
-            HU_CHECK(hipEventElapsedTime(&t01, m->ev[0], m->ev[1]));
-            HU_CHECK(hipEventElapsedTime(&t12, m->ev[1], m->ev[2]));
-            HU_CHECK(hipEventElapsedTime(&t23, m->ev[2], m->ev[3]));
-            HU_CHECK(hipEventElapsedTime(&t34, m->ev[3], m->ev[4]));
-            HU_CHECK(hipEventElapsedTime(&t45, m->ev[4], m->ev[5]));
-            if (m->tunePhase == 1) {          // this step ran the cooperative form
+            HU_CHECK(hipEventElapsedTime(&t01, e[0], e[1]));
+            HU_CHECK(hipEventElapsedTime(&t12, e[1], e[2]));
+            HU_CHECK(hipEventElapsedTime(&t23, e[2], e[3]));
+            HU_CHECK(hipEventElapsedTime(&t34, e[3], e[4]));
+            HU_CHECK(hipEventElapsedTime(&t45, e[4], e[5]));
+            if (role[b] == 1) {
                 m->tuneMs = t01 + t23;
-                m->tunePhase = 2;
-                m->tilted = false;
-            } else if (m->tunePhase == 2) {   // and this one the per-lane form
+            } else if (role[b] == 2) {
                 m->tilted = m->tuneMs < t01 + t23;
                 m->tunePhase = 0;
                 m->nextTune = m->steps + kRetune;
             }
-            if (sorted) { HU_CHECK(hipEventElapsedTime(&t67, m->ev[6], m->ev[7])); ++nsorted; }
+            if (sorted[b]) HU_CHECK(hipEventElapsedTime(&t67, e[6], e[7]));
             acc[0] += t01 + t23;
             acc[1] += t12 + t34;
             acc[2] += t45;
             acc[3] += t67;
+        };
+        for (int k = 0; k < nsteps; ++k) {
+            const int b = k & 1;
+            sorted[b] = m->n > 0 && m->p.sortEvery > 0 && m->steps % m->p.sortEvery == 0;
+            role[b] = 0;
+            if (m->n > 0 && m->tunePhase == 0 && m->steps >= m->nextTune) {
+                m->tunePhase = 1;           // this step runs the cooperative form
+                m->tilted = true;
+                role[b] = 1;
+            } else if (m->tunePhase == 1) {
+                m->tunePhase = 2;           // and this one the per-lane form
+                m->tilted = false;
+                role[b] = 2;
+            }
+            if (m->f32) one_step<float>(m, true, b); else one_step<double>(m, true, b);
+            if (k > 0) read(b ^ 1);
         }
+        if (nsteps > 0) read((nsteps - 1) & 1);
         for (int k = 0; k < 4; ++k) m->lastMs[k] = nsteps ? acc[k] / nsteps : 0;
         check_state(m);
         return CSRST_OK;
