@@ -2,6 +2,7 @@
 // main.cpp:111-136 and the Distribution generator of Distribution.cpp:55-130.
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "../../csrst_amd/host/csrst.hpp"
 
@@ -51,6 +52,32 @@ int main()
         CHECK(csrst::Distribution::radical_inverse(3, 2) == 0.75);
         CHECK(std::fabs(csrst::Distribution::radical_inverse(1, 3) - 1.0 / 3) < 1e-16);
         CHECK(std::fabs(csrst::Distribution::radical_inverse(5, 3) - 7.0 / 9) < 1e-15);   // 5 = 12_3 -> 0.21_3
+    }
+    {   // van der Corput: the first b^m - 1 radical inverses in base b are the
+        // fractions j / b^m, j = 1 .. b^m - 1, each once (digit reversal is a
+        // bijection); an index offset, a dropped or reordered digit, or a base
+        // mix-up breaks it.  Exact in binary for base 2; to 1e-15 for base 3.
+        for (unsigned b : {2u, 3u}) {
+            const unsigned long long N = b == 2 ? 4096 : 2187;   // 2^12, 3^7
+            std::vector<int> seen(N, 0);
+            bool ok = true;
+            for (unsigned long long k = 1; k < N; ++k) {
+                const double v = csrst::Distribution::radical_inverse(k, b) * (double)N;
+                const long long j = std::llround(v);
+                ok = ok && std::fabs(v - (double)j) < 1e-9 && j >= 1 && j < (long long)N && !seen[j];
+                if (j >= 1 && j < (long long)N) seen[j] = 1;
+            }
+            CHECK(ok);
+        }
+        // the digit-reversal formula itself, for indices with many digits
+        for (unsigned long long k : {12345ull, 987654321ull, (1ull << 40) + 12345ull}) {
+            for (unsigned b : {2u, 3u}) {
+                unsigned long long kk = k, num = 0, den = 1;
+                while (kk) { num = num * b + kk % b; den *= b; kk /= b; }
+                const double want = (double)num / (double)den;
+                CHECK(std::fabs(csrst::Distribution::radical_inverse(k, b) - want) <= 4e-16);
+            }
+        }
     }
     {   // generated bunch: moments, charge, momentum offset and the XFirst order of R
         csrst::Commands c;
