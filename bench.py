@@ -20,7 +20,11 @@ import subprocess
 import sys
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the whole default run (set-up, both precisions, both slices, the CPU sample)
+# is meant to finish well inside a driver's few-minute window
+DRIVER_WINDOW_S = 600.0
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
@@ -519,6 +523,8 @@ def main():
         "fp64": rec64,
         "cpu_baseline": cpu,
     }
+    rec["run_wall_s"] = time.perf_counter() - T_START
+    rec["fits_in_driver_run"] = rec["run_wall_s"] < DRIVER_WINDOW_S
     print(json.dumps(rec), flush=True)
 
 
